@@ -1,0 +1,216 @@
+/*
+ * zeno_hip.h — C ABI of the MI355X-native zeno state-vector backend.
+ *
+ * Two layers, both plain C (pointers and sizes, no C++ or torch types):
+ *
+ *  1. zb_*  — the drop-in boundary.  One entry point per method of the
+ *             reference's `pub trait Backend: Send` (src/exec.rs:134-162)
+ *             plus its constructor make_backend (src/exec.rs:306-331).  A
+ *             Rust FFI crate binds these 1:1 (binding in INTEGRATION.md).
+ *  2. zr_*  — the host side above the boundary (compiler + executor),
+ *             mirroring src/compiler.rs:118-200 compile() and
+ *             src/exec.rs:334-452 run_compiled/run_sampled.  The Python
+ *             package zeno_amd binds these through ctypes.
+ *
+ * Conventions (src/ir.rs:7-11, src/exec.rs:134-162):
+ *  - qubits are little-endian: bit q of a basis index is qubit q;
+ *  - `qs` passed to zb_apply_* is sorted ascending and distinct; bit b of a
+ *    local matrix index is qs[b];
+ *  - matrices are row-major 2^k x 2^k of complex128, interleaved (re, im)
+ *    exactly as a Rust `&[Complex64]` is laid out; diagonals have 2^k
+ *    entries;
+ *  - the backend converts f64 inputs to its precision with round to nearest
+ *    (T::fr, src/state.rs:41-50);
+ *  - ops are asynchronous on the backend's HIP stream; every read
+ *    (prob_one, measure, sample, statevector, norm_sqr, reset_all, finish)
+ *    synchronises, as the Metal backend does (src/metal.rs:562-626).
+ *
+ * Errors: functions returning int give 0 on success and a negative code on
+ * failure; functions returning double give NaN on failure.  zb_last_error()
+ * holds the message (thread-local).
+ */
+#ifndef ZENO_HIP_H
+#define ZENO_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ZB_OK 0
+#define ZB_ERR_ARG -1         /* invalid argument (qubit range, k, order) */
+#define ZB_ERR_UNSUPPORTED -2 /* Error::Unsupported (src/lib.rs:101-102) */
+#define ZB_ERR_MEMORY -3      /* Error::Memory (src/lib.rs:82-95)        */
+#define ZB_ERR_DEVICE -4      /* HIP runtime failure                     */
+
+typedef struct zb_backend zb_backend;
+
+/* ---------------------------------------------------------------------
+ * Layer 1: the Backend trait (src/exec.rs:134-162)
+ * ------------------------------------------------------------------- */
+
+/* make_backend(n_qubits, precision, BackendChoice::Hip)
+ * (src/exec.rs:306-331) + StateVec::zero_state (src/state.rs:109-117).
+ * precision: 32 or 64.  Allocates the split re/im state in HBM and sets
+ * |0...0>.  Fails with ZB_ERR_MEMORY when 2^n * 2 * sizeof(T) exceeds the
+ * free device memory, ZB_ERR_UNSUPPORTED when no HIP device is present or
+ * n > 35. */
+int zb_create(uint32_t n_qubits, int precision, zb_backend **out);
+
+/* Drop for Box<dyn Backend>: frees device memory. */
+void zb_destroy(zb_backend *h);
+
+/* Backend::name (src/exec.rs:135): "hip-f64" or "hip-f32". */
+const char *zb_name(const zb_backend *h);
+
+/* Backend::apply_unitary (src/exec.rs:136).  k in 1..6. */
+int zb_apply_unitary(zb_backend *h, const uint32_t *qs, uint32_t k,
+                     const double *mat);
+
+/* Backend::apply_diagonal (src/exec.rs:137).  k in 1..12. */
+int zb_apply_diagonal(zb_backend *h, const uint32_t *qs, uint32_t k,
+                      const double *diag);
+
+/* Backend::apply_cx (src/exec.rs:140-147; CpuBackend override at
+ * src/exec.rs:197-199 → src/state.rs:394-417): a zero-arithmetic swap over
+ * the control=1 half. */
+int zb_apply_cx(zb_backend *h, uint32_t control, uint32_t target);
+
+/* Backend::prob_one (src/exec.rs:150; src/state.rs:598-626), f64
+ * accumulation in a fixed order (4096-element blocks: 16 serial adds per
+ * lane, then a pairwise tree over 256 lanes, recursively). */
+double zb_prob_one(zb_backend *h, uint32_t q);
+
+/* Backend::measure (src/exec.rs:152; CpuBackend at :201-207): outcome =
+ * u < P(1); collapses with scale T::fr(1/sqrt(p)).  Returns 0/1 or < 0. */
+int zb_measure(zb_backend *h, uint32_t q, double u);
+
+/* Backend::reset_all (src/exec.rs:153; src/state.rs:128-132). */
+int zb_reset_all(zb_backend *h);
+
+/* Backend::sample (src/exec.rs:154; src/sample.rs:29-111) with
+ * rng = Xoshiro256PlusPlus::seed_from_u64(seed).  Writes `shots` basis
+ * indices in ascending order to `out` (host memory). */
+int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out);
+
+/* Backend::statevector (src/exec.rs:155; src/state.rs:134-140): writes
+ * 2^n interleaved complex128 values to `out` (host memory). */
+int zb_statevector(zb_backend *h, double *out);
+
+/* Backend::norm_sqr (src/exec.rs:156; src/state.rs:142-156), f64, in the
+ * fixed order of zb_prob_one. */
+double zb_norm_sqr(zb_backend *h);
+
+/* Backend::finish (src/exec.rs:157-161): waits for queued work. */
+int zb_finish(zb_backend *h);
+
+/* Thread-local message of the last failure. */
+const char *zb_last_error(void);
+
+/* ---------------------------------------------------------------------
+ * Layer 1 extensions (no counterpart method in the reference trait).
+ * ------------------------------------------------------------------- */
+
+/* Raw split state copy to host (T = float or double by precision). */
+int zb_state_raw(zb_backend *h, void *re, void *im);
+/* Raw split state upload from host. */
+int zb_state_set_raw(zb_backend *h, const void *re, const void *im);
+/* Device addresses of the split state (for zero-copy consumers). */
+int zb_state_device_ptrs(zb_backend *h, void **re, void **im);
+/* Number of qubits and precision of a handle. */
+uint32_t zb_n_qubits(const zb_backend *h);
+int zb_precision(const zb_backend *h);
+/* Per-chunk probability masses of the sampler's pass 1
+ * (src/sample.rs:38-50; chunk = min(2^n, 65536)), f64, to host. */
+int zb_chunk_sums(zb_backend *h, double *out);
+/* Execution mode: 0 = exact (one sweep per op, reference expression
+ * trees, bit-identical to the CPU path); 1 = tiled (ops whose support fits
+ * one on-chip tile are batched into a single HBM pass with the same
+ * per-op arithmetic, still bit-identical).  Default 1. */
+int zb_set_mode(zb_backend *h, int mode);
+/* The HIP stream the backend's kernels run on (hipStream_t). */
+void *zb_stream(zb_backend *h);
+/* max_i |a_i - b_i| over two states of equal size (any precisions), f64,
+ * computed on the device; used to state the f32 error against f64. */
+int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out);
+
+/* ---------------------------------------------------------------------
+ * Layer 2: compiler + executor (the host side above the boundary)
+ * ------------------------------------------------------------------- */
+
+/* Program instruction (one ir::Instr, src/ir.rs:36-52).  kind: 0 gate,
+ * 1 measure, 2 reset, 3 barrier.  cond_creg >= 0 wraps the instruction in
+ * `if (creg == cond_value)`.  72 bytes. */
+typedef struct {
+    char name[8];
+    int32_t kind;
+    int32_t nq;
+    int32_t np;
+    uint32_t q[3];
+    uint32_t clbit;
+    int32_t cond_creg;
+    uint64_t cond_value;
+    double p[3];
+} zr_instr;
+
+/* noise::NoiseModel (src/noise.rs:37-46). */
+typedef struct {
+    double depolarizing_1q, depolarizing_2q, bit_flip, phase_flip,
+        amplitude_damping, readout_flip_0to1, readout_flip_1to0;
+} zr_noise;
+
+typedef struct zr_compiled zr_compiled;
+
+/* compiler::compile (src/compiler.rs:118-200): cancel, lower, diagonal
+ * fusion (diag_max <= 12), dense fusion (fusion_max <= 6).  Returns
+ * NULL on error (zb_last_error()). */
+zr_compiled *zr_compile(const zr_instr *prog, int64_t n_instr,
+                        uint32_t n_qubits, const uint32_t *creg_sizes,
+                        uint32_t n_cregs, int fusion_max, int diag_max);
+void zr_compiled_free(zr_compiled *c);
+int64_t zr_compiled_nops(const zr_compiled *c);
+/* Op kinds: 0 unitary, 1 diagonal, 2 cx, 3 measure, 4 reset.  data gets
+ * the matrix or diagonal (interleaved complex128); cond = offset, len,
+ * value of an `if` wrapper. */
+int zr_compiled_op(const zr_compiled *c, int64_t i, int32_t *kind,
+                   int32_t *k, uint32_t *qs, double *data, int32_t *is_if,
+                   uint64_t *cond);
+/* compiler::Stats (src/compiler.rs:68-78) + dynamic + #final measures. */
+void zr_compiled_stats(const zr_compiled *c, int64_t *stats /* 6 */);
+int64_t zr_compiled_final_measures(const zr_compiled *c, uint32_t *pairs);
+
+/* Apply ops [lo, hi) of a compiled program to a backend
+ * (src/exec.rs:398-405 apply_op in a loop). */
+int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi);
+
+/* exec::run_compiled (src/exec.rs:334-549) on the HIP backend: static
+ * circuits run once and are sampled (run_sampled); dynamic or noisy ones
+ * run per shot (run_per_shot).  Writes the unique clbit words with their
+ * counts, ascending by word; returns how many, or < 0.  sim_seconds covers
+ * state allocation, every op, finish and sampling (src/exec.rs:372-381).
+ * mode: see zb_set_mode. */
+int64_t zr_run(const zr_compiled *c, int precision, uint64_t shots,
+               uint64_t seed, const zr_noise *noise, int mode,
+               uint64_t *keys, uint64_t *counts, int64_t cap,
+               double *sim_seconds);
+
+/* Seeded circuit generators (src/circuit.rs:138-174 and the benchmark
+ * configurations' matching/RZZ variant).  Return the instruction count;
+ * write at most `cap`. */
+int64_t zr_gen_random(uint32_t n, uint32_t depth, uint64_t seed,
+                      zr_instr *out, int64_t cap);
+int64_t zr_gen_qft(uint32_t n, zr_instr *out, int64_t cap);
+int64_t zr_gen_u3_qft(uint32_t n, uint64_t seed, zr_instr *out, int64_t cap);
+int64_t zr_gen_matching(uint32_t n, uint32_t depth, uint64_t seed,
+                        uint32_t rzz_period, zr_instr *out, int64_t cap);
+
+/* gates::build (src/gates.rs:119-188): returns the arity or -1. */
+int zr_gate_build(const char *name, const double *params, int np,
+                  double *out, int *is_diag);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

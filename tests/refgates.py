@@ -149,8 +149,9 @@ NATIVE = [
 
 
 class TestRng:
-    __test__ = False
     """splitmix64 test RNG of tests/reference.rs:36-64."""
+
+    __test__ = False
 
     M = (1 << 64) - 1
 

@@ -1,0 +1,217 @@
+"""GPU parity of the HIP backend against the CPU oracle, through the C ABI.
+
+Bar: bit-exact for every sweep (the backend reproduces the reference
+expression trees without FMA contraction), for CX (a pure permutation), for
+prob_one / norm_sqr (same fixed reduction order), for the sampler's indices
+and hence for counts.  At sizes the oracle cannot finish quickly, the tests
+use size-independent properties: norm, forward-then-adjoint returns |0>,
+and the QFT's closed form.
+"""
+import math
+import sys
+
+import numpy as np
+import pytest
+
+import zeno_amd as z
+from circuits import diag_ladder, disjoint_blocks, random_mixed
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True, file=sys.stderr)
+
+
+def _rand_state(n, prec, seed):
+    rng = np.random.default_rng(seed)
+    dt = np.float64 if prec == 64 else np.float32
+    v = rng.normal(size=(2, 1 << n))
+    v /= np.sqrt(np.sum(v**2))
+    return v[0].astype(dt), v[1].astype(dt)
+
+
+def _bits(a):
+    return a.view(np.uint64 if a.dtype == np.float64 else np.uint32)
+
+
+def _assert_same(gpu, ora, what):
+    gr, gi = gpu.raw()
+    orr, oi = ora.raw()
+    ok = np.array_equal(_bits(gr), _bits(orr)) and np.array_equal(_bits(gi), _bits(oi))
+    if not ok:
+        d = max(np.max(np.abs(gr.astype(np.float64) - orr)), np.max(np.abs(gi.astype(np.float64) - oi)))
+        raise AssertionError(f"{what}: not bit-identical (max |d| = {d:.3e})")
+
+
+def _pair(orc, n, prec, seed):
+    re, im = _rand_state(n, prec, seed)
+    g = z.HipBackend(n, prec, z.HipBackend.EXACT)
+    g.set_raw(re, im)
+    o = orc.State(n, prec)
+    o.set_raw(re, im)
+    return g, o
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_single_ops_bit_exact(orc, prec):
+    n = 13
+    rng = np.random.default_rng(1)
+    g, o = _pair(orc, n, prec, 11)
+    for q in range(n):  # 1q on every target
+        m = rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))
+        _say(f"1q prec={prec} q={q}")
+        g.apply_unitary([q], m)
+        o.apply_unitary([q], m)
+    _assert_same(g, o, "1q sweep")
+    for k in range(2, 7):  # dense kq, low/high/mixed supports
+        for qs in ([0, 1, 2, 3, 4, 5][:k], list(range(n - k, n)), sorted(rng.choice(n, k, replace=False).tolist())):
+            m = rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k))
+            _say(f"kq prec={prec} k={k} qs={qs}")
+            g.apply_unitary(qs, m)
+            o.apply_unitary(qs, m)
+            _assert_same(g, o, f"kq k={k} qs={qs}")
+    for k in (1, 2, 3, 5, 8, 10, 12):
+        qs = sorted(rng.choice(n, k, replace=False).tolist())
+        d = np.exp(1j * rng.uniform(0, 2 * np.pi, 1 << k)) * rng.uniform(0.5, 1.5, 1 << k)
+        _say(f"diag prec={prec} k={k} qs={qs}")
+        g.apply_diagonal(qs, d)
+        o.apply_diagonal(qs, d)
+        _assert_same(g, o, f"diag k={k}")
+    for c, t in [(0, 1), (1, 0), (0, n - 1), (n - 1, 0), (5, 9), (9, 5)]:
+        _say(f"cx prec={prec} c={c} t={t}")
+        g.apply_cx(c, t)
+        o.apply_cx(c, t)
+    _assert_same(g, o, "cx")
+    assert g.norm_sqr() == o.norm_sqr()
+    for q in range(n):
+        assert g.prob_one(q) == o.prob_one(q), q
+    g.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_random_circuits_bit_exact(orc, prec):
+    for n in (2, 3, 5, 8, 11, 14, 17, 20):
+        prog = orc.gen_random(n, 12, 1000 + n)
+        for fusion in (0, 1, 2, 4, 6):
+            _say(f"random_circuit n={n} fusion={fusion} prec={prec}")
+            comp = orc.Compiled(prog, n, fusion=fusion)
+            o = orc.State(n, prec)
+            o.apply_compiled(comp)
+            with z.HipBackend(n, prec, z.HipBackend.EXACT) as g:
+                g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                _assert_same(g, o, f"n={n} fusion={fusion}")
+
+
+def test_adversarial_circuits_bit_exact(orc):
+    cases = [(random_mixed(orc, n, 80, 9000 + n), n) for n in (3, 6, 10, 13)]
+    cases += [(diag_ladder(orc), 6), (disjoint_blocks(orc), 8), (orc.gen_u3_qft(12, 5), 12),
+              (orc.gen_matching(16, 10, 3, 5), 16)]
+    for prog, n in cases:
+        for fusion in (0, 1, 3, 5):
+            _say(f"adversarial n={n} fusion={fusion}")
+            o = orc.State(n, 64)
+            o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+            with z.HipBackend(n, 64, z.HipBackend.EXACT) as g:
+                g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                _assert_same(g, o, f"n={n} fusion={fusion}")
+
+
+def test_sampler_indices_and_chunk_sums_bit_exact(orc):
+    for n, shots, seed in ((1, 100, 3), (4, 1000, 1), (10, 5000, 2), (16, 20000, 5), (17, 4096, 3), (20, 100000, 4)):
+        _say(f"sample n={n} shots={shots}")
+        prog = random_mixed(orc, n, 3 * n, 700 + n)
+        o = orc.State(n, 64)
+        o.apply_compiled(orc.Compiled(prog, n, fusion=3))
+        with z.HipBackend(n, 64) as g:
+            g.apply_compiled(z.compile(prog, n, fusion=3))
+            assert np.array_equal(g.chunk_sums().view(np.uint64), o.chunk_sums().view(np.uint64))
+            assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed))
+
+
+def test_measure_and_collapse_bit_exact(orc):
+    n = 12
+    for prec in (64, 32):
+        g, o = _pair(orc, n, prec, 21)
+        for i, (q, u) in enumerate([(0, 0.3), (5, 0.9), (11, 0.1), (3, 0.5)]):
+            _say(f"measure prec={prec} q={q}")
+            assert g.measure(q, u) == o.measure(q, u)
+            _assert_same(g, o, f"collapse #{i}")
+        g.reset_all()
+        o.reset_all()
+        _assert_same(g, o, "reset")
+        g.close()
+
+
+def test_c1_counts_identical(orc):
+    """20 qubits, depth 12, fusion 2, 1024 shots, seed 42 — both the
+    reference brickwork and the random-matching variant."""
+    for prog in (orc.gen_random(20, 12, 42), orc.gen_matching(20, 12, 42)):
+        full = orc.program([prog, orc.measure_all(20)])
+        _say("C1 counts")
+        want, _ = orc.run(orc.Compiled(full, 20, fusion=2), 1024, 42)
+        r = z.run_program(full, 20, opts=z.RunOptions(shots=1024, seed=42, precision=64, fusion_max=2))
+        assert r.raw_counts == want
+        assert r.counts.total() == 1024 and r.backend == "hip-f64"
+
+
+def test_dynamic_and_noisy_runs_match_oracle(orc):
+    I = orc.instr
+    rows = [I("x", [0]), I("h", [1]), I("cx", [1, 2]), I("cx", [0, 1]), I("h", [0]),
+            orc.measure(0, 0), orc.measure(1, 1), I("x", [2], cond=(1, 1)), I("z", [2], cond=(0, 1)),
+            orc.measure(2, 2)]
+    prog = orc.program(rows)
+    _say("teleport per-shot")
+    want, _ = orc.run(orc.Compiled(prog, 3, cregs=[1, 1, 1]), 200, 5)
+    got = z.run_program(prog, 3, cregs=[1, 1, 1], opts=z.RunOptions(shots=200, seed=5))
+    assert got.raw_counts == want
+    nz = {"depolarizing_1q": 0.02, "depolarizing_2q": 0.05, "amplitude_damping": 0.03,
+          "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+    full = orc.program([orc.gen_random(5, 4, 9), orc.measure_all(5)])
+    for prec in (64, 32):
+        _say(f"noisy per-shot prec={prec}")
+        want, _ = orc.run(orc.Compiled(full, 5, fusion=0), 64, 17, prec=prec, noise=nz)
+        got = z.run_program(full, 5, opts=z.RunOptions(shots=64, seed=17, precision=prec, noise=z.NoiseModel(**nz)))
+        assert got.raw_counts == want
+
+
+def _adjoint_roundtrip(n, prec, prog, fusion, tol):
+    comp = z.compile(prog, n, fusion=fusion)
+    with z.HipBackend(n, prec) as g:
+        g.apply_compiled(comp)
+        nrm = g.norm_sqr()
+        for op in reversed(comp.ops()):
+            if op["kind"] == z.exec.OP_U:
+                g.apply_unitary(op["qubits"], op["mat"].conj().T)
+            elif op["kind"] == z.exec.OP_D:
+                g.apply_diagonal(op["qubits"], op["diag"].conj())
+            else:
+                g.apply_cx(*op["qubits"])
+        p0 = 1.0 - max(g.prob_one(q) for q in range(n))
+        assert abs(nrm - 1) < tol, nrm
+        assert p0 > 1 - tol, p0
+
+
+def test_large_state_roundtrip_and_norm():
+    n = 26
+    _say("roundtrip 26q f64 fusion=4")
+    _adjoint_roundtrip(n, 64, z.matching_circuit(n, 6, 11, 5).to_program()[0], 4, 1e-9)
+    _say("roundtrip 26q f32 fusion=5")
+    _adjoint_roundtrip(n, 32, z.matching_circuit(n, 6, 11, 5).to_program()[0], 5, 1e-4)
+
+
+def test_qft24_closed_form():
+    n, x = 24, 0x5A5A5A
+    rows = [z.ir.instr("x", [q]) for q in range(n) if (x >> q) & 1]
+    prog = z.ir.program(rows + [z.qft(n).to_program()[0]])
+    for fusion in (1, 4):
+        _say(f"qft24 fusion={fusion}")
+        with z.HipBackend(n, 64) as g:
+            g.apply_compiled(z.compile(prog, n, fusion=fusion))
+            sv = g.statevector()
+        j = np.arange(1 << n, dtype=np.int64)
+        rev = np.zeros_like(j)
+        for b in range(n):
+            rev |= ((j >> b) & 1) << (n - 1 - b)
+        want = np.exp(2j * np.pi * ((x * j) % (1 << n)) / (1 << n)) / math.sqrt(1 << n)
+        assert np.max(np.abs(sv[rev] - want)) < 1e-10

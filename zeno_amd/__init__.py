@@ -1,0 +1,34 @@
+"""zeno_amd — an MI355X-native (gfx950 HIP) state-vector backend with the
+capabilities of cleoanka/zeno's CPU path.
+
+The compute lives in zeno_amd/libzeno_hip.so (HIP kernels + the C++ host
+side, C ABI in include/zeno_hip.h).  Importing this package does not touch
+the GPU; constructing a backend or running a circuit does, and fails loudly
+if the library or a HIP device is missing.
+"""
+from .backend import DeviceError, HipBackend, MemoryBudget, Unsupported, ZenoError
+from .circuit import Circuit, matching_circuit, qft, random_circuit, u3_qft
+from .exec import (
+    Compiled,
+    Counts,
+    InvalidCircuit,
+    NoiseModel,
+    RunOptions,
+    RunResult,
+    Simulator,
+    compile,
+    format_key,
+    max_qubits,
+    plan,
+    run_compiled,
+    run_program,
+    state_bytes,
+)
+from . import ir
+
+__all__ = [
+    "Circuit", "Compiled", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
+    "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
+    "format_key", "ir", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
+    "run_compiled", "run_program", "state_bytes", "u3_qft",
+]

@@ -1,0 +1,553 @@
+// backend.hip — the HIP implementation of the reference's Backend trait
+// (src/exec.rs:134-162) behind the C ABI of include/zeno_hip.h.
+//
+// One handle owns one split-complex state in HBM and one HIP stream.  Ops
+// are enqueued asynchronously; reads synchronise.  Matrices and diagonal
+// tables travel through a pinned-host -> device parameter arena that is
+// only reused after a stream synchronisation, so the caller's buffers are
+// borrowed for the call only (as src/metal.rs:365-367 requires).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <limits>
+#include <string>
+#include <vector>
+
+#include "host.hpp"
+#include "kernels.hpp"
+#include "zeno_hip.h"
+
+using namespace zeno;
+
+namespace {
+
+constexpr size_t kArenaBytes = 64ull << 20;
+
+struct HipError {
+    std::string msg;
+};
+
+inline void check(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw HipError{std::string(what) + ": " + hipGetErrorString(e)};
+}
+
+unsigned grid_for(uint64_t work) {
+    const uint64_t blocks = (work + dev::kBlock - 1) / dev::kBlock;
+    return (unsigned)std::max<uint64_t>(1, std::min<uint64_t>(blocks, 65536));
+}
+
+}  // namespace
+
+struct zb_backend {
+    uint32_t n = 0;
+    int prec = 64;
+    uint64_t len = 0;
+    size_t elem = 8;
+    void *re = nullptr, *im = nullptr;
+    hipStream_t stream = nullptr;
+    int mode = 1;
+    std::string name;
+    // parameter arena
+    char *arena_dev = nullptr, *arena_host = nullptr;
+    size_t arena_off = 0;
+    // reduction scratch (device) + pinned result slot
+    double *red_dev = nullptr;
+    size_t red_cap = 0;
+    double *red_host = nullptr;
+    // grow-only scratch for the sampler
+    char *scratch = nullptr;
+    size_t scratch_cap = 0;
+
+    void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+    // Copy `bytes` of host data into the arena; returns the device address.
+    void *upload(const void *src, size_t bytes) {
+        const size_t need = (bytes + 255) & ~size_t(255);
+        if (arena_off + need > kArenaBytes) {
+            sync();
+            arena_off = 0;
+        }
+        std::memcpy(arena_host + arena_off, src, bytes);
+        void *d = arena_dev + arena_off;
+        check(hipMemcpyAsync(d, arena_host + arena_off, bytes, hipMemcpyHostToDevice, stream), "param upload");
+        arena_off += need;
+        return d;
+    }
+
+    void ensure_red(size_t n_doubles) {
+        if (n_doubles <= red_cap) return;
+        sync();
+        if (red_dev) check(hipFree(red_dev), "hipFree");
+        red_cap = std::max<size_t>(n_doubles, 1 << 16);
+        check(hipMalloc(&red_dev, red_cap * sizeof(double)), "hipMalloc reduce");
+    }
+
+    char *scratch_get(size_t bytes) {
+        if (bytes > scratch_cap) {
+            sync();
+            if (scratch) check(hipFree(scratch), "hipFree");
+            scratch = nullptr;
+            scratch_cap = std::max<size_t>(bytes, 1 << 20);
+            check(hipMalloc((void **)&scratch, scratch_cap), "hipMalloc scratch");
+        }
+        return scratch;
+    }
+
+    template <typename T>
+    T *R() {
+        return static_cast<T *>(re);
+    }
+    template <typename T>
+    T *I() {
+        return static_cast<T *>(im);
+    }
+};
+
+namespace {
+
+template <typename T>
+void launch_zero(zb_backend *h) {
+    hipLaunchKernelGGL(dev::k_zero<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
+                       h->len);
+    check(hipGetLastError(), "k_zero");
+}
+
+template <typename T>
+void to_T(const double *src, size_t n, std::vector<T> &re, std::vector<T> &im) {
+    re.resize(n);
+    im.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        re[i] = (T)src[2 * i];  // T::fr: round to nearest (src/state.rs:41-50)
+        im[i] = (T)src[2 * i + 1];
+    }
+}
+
+template <typename T, int K>
+void launch_kq(zb_backend *h, const T *dmat, const dev::Qubits &qs) {
+    const uint64_t groups = h->len >> K;
+    hipLaunchKernelGGL((dev::k_kq<T, K>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), dmat, qs, groups);
+    check(hipGetLastError(), "k_kq");
+}
+
+template <typename T>
+void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
+    const size_t dim = (size_t)1 << k;
+    std::vector<T> mr, mi;
+    to_T<T>(mat, dim * dim, mr, mi);
+    if (k == 1) {
+        dev::Mat2<T> m;
+        for (int i = 0; i < 4; ++i) m.r[i] = mr[i], m.i[i] = mi[i];
+        const uint64_t half = h->len >> 1;
+        hipLaunchKernelGGL(dev::k_1q<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
+                           qs[0], half, m);
+        check(hipGetLastError(), "k_1q");
+        return;
+    }
+    std::vector<T> packed(2 * dim * dim);
+    std::copy(mr.begin(), mr.end(), packed.begin());
+    std::copy(mi.begin(), mi.end(), packed.begin() + dim * dim);
+    const T *d = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
+    dev::Qubits q{};
+    q.k = (int)k;
+    for (uint32_t b = 0; b < k; ++b) q.q[b] = qs[b];
+    switch (k) {
+        case 2: launch_kq<T, 2>(h, d, q); break;
+        case 3: launch_kq<T, 3>(h, d, q); break;
+        case 4: launch_kq<T, 4>(h, d, q); break;
+        case 5: launch_kq<T, 5>(h, d, q); break;
+        case 6: launch_kq<T, 6>(h, d, q); break;
+    }
+}
+
+template <typename T>
+void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
+    const size_t dim = (size_t)1 << k;
+    std::vector<T> dr, di;
+    to_T<T>(d, dim, dr, di);
+    if (k == 1) {
+        const uint64_t half = h->len >> 1;
+        hipLaunchKernelGGL(dev::k_diag1<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
+        check(hipGetLastError(), "k_diag1");
+        return;
+    }
+    std::vector<T> packed(2 * dim);
+    std::copy(dr.begin(), dr.end(), packed.begin());
+    std::copy(di.begin(), di.end(), packed.begin() + dim);
+    const T *tab = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
+    dev::Qubits q{};
+    q.k = (int)k;
+    for (uint32_t b = 0; b < k; ++b) q.q[b] = qs[b];
+    hipLaunchKernelGGL(dev::k_diagk<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), tab, q, h->len);
+    check(hipGetLastError(), "k_diagk");
+}
+
+template <typename T>
+void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
+    const uint64_t groups = h->len >> 2;
+    hipLaunchKernelGGL(dev::k_cx<T>, dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
+                       std::min(c, t), std::max(c, t), 1ull << c, 1ull << t, groups);
+    check(hipGetLastError(), "k_cx");
+}
+
+// Fixed-order reduction (mode 0: norm, mode 1: prob_one of q).
+template <typename T>
+double reduce_T(zb_backend *h, int mode, uint32_t q) {
+    uint64_t n = mode ? h->len >> 1 : h->len;
+    uint64_t nb = (n + dev::kReduceSpan - 1) / dev::kReduceSpan;
+    h->ensure_red(2 * nb + 2);
+    double *a = h->red_dev, *b = h->red_dev + nb + 1;
+    hipLaunchKernelGGL(dev::k_reduce_state<T>, dim3((unsigned)nb), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), n, mode, q, a);
+    check(hipGetLastError(), "k_reduce_state");
+    while (nb > 1) {
+        const uint64_t nn = (nb + dev::kReduceSpan - 1) / dev::kReduceSpan;
+        hipLaunchKernelGGL(dev::k_reduce_f64, dim3((unsigned)nn), dim3(dev::kBlock), 0, h->stream, a, nb, b);
+        check(hipGetLastError(), "k_reduce_f64");
+        std::swap(a, b);
+        nb = nn;
+    }
+    check(hipMemcpyAsync(h->red_host, a, sizeof(double), hipMemcpyDeviceToHost, h->stream), "reduce copy");
+    h->sync();
+    return *h->red_host;
+}
+
+template <typename T>
+int measure_T(zb_backend *h, uint32_t q, double u) {
+    const double p1 = reduce_T<T>(h, 1, q);
+    const int outcome = u < p1;
+    const double p = outcome ? p1 : 1.0 - p1;
+    const T scale = (T)(1.0 / std::sqrt(p));
+    const uint64_t half = h->len >> 1;
+    hipLaunchKernelGGL(dev::k_collapse<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), q, half, outcome, scale);
+    check(hipGetLastError(), "k_collapse");
+    return outcome;
+}
+
+template <typename T>
+void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
+    hipLaunchKernelGGL(dev::k_chunk_sums<T>, dim3((unsigned)nch), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), h->len, chunk, dsums);
+    check(hipGetLastError(), "k_chunk_sums");
+}
+
+// sample_indices (src/sample.rs:29-111).  Passes 1 and 5-6 run on the
+// device; the serial prefix, the serial draws, the sort and the routing
+// (passes 2-4) run on the host, in the reference's order.
+template <typename T>
+void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
+    if (shots == 0) return;
+    const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
+    const uint64_t nch = (h->len + chunk - 1) / chunk;
+    double *dsums = (double *)h->scratch_get(nch * sizeof(double));
+    chunk_sums_T<T>(h, dsums, chunk, nch);
+    std::vector<double> sums(nch);
+    check(hipMemcpyAsync(sums.data(), dsums, nch * sizeof(double), hipMemcpyDeviceToHost, h->stream), "sums copy");
+    h->sync();
+    std::vector<double> cum(nch + 1, 0.0);
+    for (uint64_t c = 0; c < nch; ++c) cum[c + 1] = cum[c] + sums[c];
+    const double total = cum[nch];
+    Rng rng(seed);
+    std::vector<double> us(shots);
+    for (uint64_t i = 0; i < shots; ++i) us[i] = rng.f64() * total;
+    std::sort(us.begin(), us.end());
+    std::vector<uint64_t> starts(nch + 1);
+    for (uint64_t c = 0; c <= nch; ++c)
+        starts[c] = (uint64_t)(std::lower_bound(us.begin(), us.end(), cum[c]) - us.begin());
+    std::vector<uint32_t> touched;
+    for (uint64_t c = 0; c < nch; ++c)
+        if (starts[c] < starts[c + 1]) touched.push_back((uint32_t)c);
+    if (!touched.empty()) {
+        const size_t bytes = touched.size() * 4 + (nch + 1) * 8 * 2 + shots * 8 * 2;
+        char *p = h->scratch_get(bytes + 64);
+        double *dcum = (double *)p;
+        p += (nch + 1) * 8;
+        uint64_t *dstarts = (uint64_t *)p;
+        p += (nch + 1) * 8;
+        double *dus = (double *)p;
+        p += shots * 8;
+        uint64_t *dout = (uint64_t *)p;
+        p += shots * 8;
+        uint32_t *dtouched = (uint32_t *)p;
+        check(hipMemcpyAsync(dcum, cum.data(), (nch + 1) * 8, hipMemcpyHostToDevice, h->stream), "cum");
+        check(hipMemcpyAsync(dstarts, starts.data(), (nch + 1) * 8, hipMemcpyHostToDevice, h->stream), "starts");
+        check(hipMemcpyAsync(dus, us.data(), shots * 8, hipMemcpyHostToDevice, h->stream), "us");
+        check(hipMemcpyAsync(dtouched, touched.data(), touched.size() * 4, hipMemcpyHostToDevice, h->stream),
+              "touched");
+        hipLaunchKernelGGL(dev::k_rescan<T>, dim3((unsigned)touched.size()), dim3(dev::kBlock), 0, h->stream,
+                           h->R<T>(), h->I<T>(), h->len, chunk, dtouched, dcum, dstarts, dus, dout);
+        check(hipGetLastError(), "k_rescan");
+        check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
+    }
+    h->sync();
+}
+
+template <typename F>
+int guarded(F &&f) {
+    try {
+        f();
+        return ZB_OK;
+    } catch (const HipError &e) {
+        set_error(e.msg);
+        return ZB_ERR_DEVICE;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return ZB_ERR_ARG;
+    }
+}
+
+bool sorted_distinct(const uint32_t *qs, uint32_t k, uint32_t n) {
+    for (uint32_t b = 0; b < k; ++b) {
+        if (qs[b] >= n) return false;
+        if (b && qs[b] <= qs[b - 1]) return false;
+    }
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *zb_last_error(void) { return last_error(); }
+
+int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
+    *out = nullptr;
+    if (precision != 32 && precision != 64) {
+        set_error("precision must be 32 or 64");
+        return ZB_ERR_ARG;
+    }
+    if (n_qubits == 0 || n_qubits > 35) {
+        set_error("the HIP backend supports 1..35 qubits (got " + std::to_string(n_qubits) + ")");
+        return ZB_ERR_UNSUPPORTED;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device is available (the hip backend needs an AMD GPU)");
+        return ZB_ERR_UNSUPPORTED;
+    }
+    zb_backend *h = new zb_backend();
+    h->n = n_qubits;
+    h->prec = precision;
+    h->len = 1ull << n_qubits;
+    h->elem = precision == 64 ? 8 : 4;
+    h->name = precision == 64 ? "hip-f64" : "hip-f32";
+    const int rc = guarded([&] {
+        size_t free_b = 0, total_b = 0;
+        check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
+        const size_t need = 2 * h->len * h->elem;
+        if (need + kArenaBytes + (64ull << 20) > free_b)
+            throw std::runtime_error("state vector for " + std::to_string(n_qubits) + " qubits at f" +
+                                     std::to_string(precision) + " needs " + std::to_string(need) +
+                                     " B but only " + std::to_string(free_b) + " B of device memory are free");
+        check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipMalloc(&h->re, h->len * h->elem), "hipMalloc re");
+        check(hipMalloc(&h->im, h->len * h->elem), "hipMalloc im");
+        check(hipMalloc((void **)&h->arena_dev, kArenaBytes), "hipMalloc arena");
+        check(hipHostMalloc((void **)&h->arena_host, kArenaBytes, hipHostMallocDefault), "hipHostMalloc arena");
+        check(hipHostMalloc((void **)&h->red_host, sizeof(double), hipHostMallocDefault), "hipHostMalloc red");
+        if (precision == 64) launch_zero<double>(h);
+        else launch_zero<float>(h);
+        h->sync();
+    });
+    if (rc != ZB_OK) {
+        zb_destroy(h);
+        return std::string(last_error()).find("needs") != std::string::npos ? ZB_ERR_MEMORY : rc;
+    }
+    *out = h;
+    return ZB_OK;
+}
+
+void zb_destroy(zb_backend *h) {
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    if (h->re) (void)hipFree(h->re);
+    if (h->im) (void)hipFree(h->im);
+    if (h->arena_dev) (void)hipFree(h->arena_dev);
+    if (h->arena_host) (void)hipHostFree(h->arena_host);
+    if (h->red_dev) (void)hipFree(h->red_dev);
+    if (h->red_host) (void)hipHostFree(h->red_host);
+    if (h->scratch) (void)hipFree(h->scratch);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+}
+
+const char *zb_name(const zb_backend *h) { return h->name.c_str(); }
+uint32_t zb_n_qubits(const zb_backend *h) { return h->n; }
+int zb_precision(const zb_backend *h) { return h->prec; }
+void *zb_stream(zb_backend *h) { return (void *)h->stream; }
+
+int zb_set_mode(zb_backend *h, int mode) {
+    if (mode != 0 && mode != 1) {
+        set_error("mode must be 0 (exact) or 1 (tiled)");
+        return ZB_ERR_ARG;
+    }
+    h->mode = mode;
+    return ZB_OK;
+}
+
+int zb_apply_unitary(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
+    if (k < 1 || k > (uint32_t)dev::kMaxFusedK || !sorted_distinct(qs, k, h->n)) {
+        set_error("apply_unitary: qubits must be sorted, distinct, < n, and 1 <= k <= 6");
+        return ZB_ERR_ARG;
+    }
+    return guarded([&] {
+        if (h->prec == 64) apply_unitary_T<double>(h, qs, k, mat);
+        else apply_unitary_T<float>(h, qs, k, mat);
+    });
+}
+
+int zb_apply_diagonal(zb_backend *h, const uint32_t *qs, uint32_t k, const double *diag) {
+    if (k < 1 || k > (uint32_t)dev::kMaxDiagK || !sorted_distinct(qs, k, h->n)) {
+        set_error("apply_diagonal: qubits must be sorted, distinct, < n, and 1 <= k <= 12");
+        return ZB_ERR_ARG;
+    }
+    return guarded([&] {
+        if (h->prec == 64) apply_diag_T<double>(h, qs, k, diag);
+        else apply_diag_T<float>(h, qs, k, diag);
+    });
+}
+
+int zb_apply_cx(zb_backend *h, uint32_t control, uint32_t target) {
+    if (control >= h->n || target >= h->n || control == target) {
+        set_error("apply_cx: control and target must be distinct qubits < n");
+        return ZB_ERR_ARG;
+    }
+    return guarded([&] {
+        if (h->prec == 64) apply_cx_T<double>(h, control, target);
+        else apply_cx_T<float>(h, control, target);
+    });
+}
+
+double zb_prob_one(zb_backend *h, uint32_t q) {
+    if (q >= h->n) {
+        set_error("prob_one: qubit out of range");
+        return std::numeric_limits<double>::quiet_NaN();
+    }
+    double r = std::numeric_limits<double>::quiet_NaN();
+    guarded([&] { r = h->prec == 64 ? reduce_T<double>(h, 1, q) : reduce_T<float>(h, 1, q); });
+    return r;
+}
+
+int zb_measure(zb_backend *h, uint32_t q, double u) {
+    if (q >= h->n) {
+        set_error("measure: qubit out of range");
+        return ZB_ERR_ARG;
+    }
+    int out = 0;
+    const int rc = guarded([&] { out = h->prec == 64 ? measure_T<double>(h, q, u) : measure_T<float>(h, q, u); });
+    return rc == ZB_OK ? out : rc;
+}
+
+int zb_reset_all(zb_backend *h) {
+    return guarded([&] {
+        if (h->prec == 64) launch_zero<double>(h);
+        else launch_zero<float>(h);
+        h->sync();
+    });
+}
+
+int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
+    return guarded([&] {
+        if (h->prec == 64) sample_T<double>(h, shots, seed, out);
+        else sample_T<float>(h, shots, seed, out);
+    });
+}
+
+int zb_chunk_sums(zb_backend *h, double *out) {
+    return guarded([&] {
+        const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
+        const uint64_t nch = (h->len + chunk - 1) / chunk;
+        double *d = (double *)h->scratch_get(nch * 8);
+        if (h->prec == 64) chunk_sums_T<double>(h, d, chunk, nch);
+        else chunk_sums_T<float>(h, d, chunk, nch);
+        check(hipMemcpyAsync(out, d, nch * 8, hipMemcpyDeviceToHost, h->stream), "copy");
+        h->sync();
+    });
+}
+
+int zb_state_raw(zb_backend *h, void *re, void *im) {
+    return guarded([&] {
+        check(hipMemcpyAsync(re, h->re, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy re");
+        check(hipMemcpyAsync(im, h->im, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy im");
+        h->sync();
+    });
+}
+
+int zb_state_set_raw(zb_backend *h, const void *re, const void *im) {
+    return guarded([&] {
+        check(hipMemcpyAsync(h->re, re, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy re");
+        check(hipMemcpyAsync(h->im, im, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy im");
+        h->sync();
+    });
+}
+
+int zb_state_device_ptrs(zb_backend *h, void **re, void **im) {
+    *re = h->re;
+    *im = h->im;
+    return ZB_OK;
+}
+
+int zb_statevector(zb_backend *h, double *out) {
+    return guarded([&] {
+        const uint64_t n = h->len;
+        if (h->prec == 64) {
+            std::vector<double> re(n), im(n);
+            check(hipMemcpyAsync(re.data(), h->re, n * 8, hipMemcpyDeviceToHost, h->stream), "copy re");
+            check(hipMemcpyAsync(im.data(), h->im, n * 8, hipMemcpyDeviceToHost, h->stream), "copy im");
+            h->sync();
+            for (uint64_t i = 0; i < n; ++i) out[2 * i] = re[i], out[2 * i + 1] = im[i];
+        } else {
+            std::vector<float> re(n), im(n);
+            check(hipMemcpyAsync(re.data(), h->re, n * 4, hipMemcpyDeviceToHost, h->stream), "copy re");
+            check(hipMemcpyAsync(im.data(), h->im, n * 4, hipMemcpyDeviceToHost, h->stream), "copy im");
+            h->sync();
+            for (uint64_t i = 0; i < n; ++i) out[2 * i] = (double)re[i], out[2 * i + 1] = (double)im[i];
+        }
+    });
+}
+
+double zb_norm_sqr(zb_backend *h) {
+    double r = std::numeric_limits<double>::quiet_NaN();
+    guarded([&] { r = h->prec == 64 ? reduce_T<double>(h, 0, 0) : reduce_T<float>(h, 0, 0); });
+    return r;
+}
+
+int zb_finish(zb_backend *h) {
+    return guarded([&] { h->sync(); });
+}
+
+int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out) {
+    if (a->n != b->n) {
+        set_error("max_abs_diff: states have different qubit counts");
+        return ZB_ERR_ARG;
+    }
+    return guarded([&] {
+        b->sync();
+        unsigned long long *d = reinterpret_cast<unsigned long long *>(a->scratch_get(8));
+        check(hipMemsetAsync(d, 0, 8, a->stream), "hipMemsetAsync");
+        const dim3 g(grid_for(a->len)), blk(dev::kBlock);
+        if (a->prec == 64 && b->prec == 64)
+            hipLaunchKernelGGL((dev::k_max_abs_diff<double, double>), g, blk, 0, a->stream, a->R<double>(),
+                               a->I<double>(), b->R<double>(), b->I<double>(), a->len, d);
+        else if (a->prec == 64)
+            hipLaunchKernelGGL((dev::k_max_abs_diff<double, float>), g, blk, 0, a->stream, a->R<double>(),
+                               a->I<double>(), b->R<float>(), b->I<float>(), a->len, d);
+        else if (b->prec == 64)
+            hipLaunchKernelGGL((dev::k_max_abs_diff<float, double>), g, blk, 0, a->stream, a->R<float>(),
+                               a->I<float>(), b->R<double>(), b->I<double>(), a->len, d);
+        else
+            hipLaunchKernelGGL((dev::k_max_abs_diff<float, float>), g, blk, 0, a->stream, a->R<float>(),
+                               a->I<float>(), b->R<float>(), b->I<float>(), a->len, d);
+        check(hipGetLastError(), "k_max_abs_diff");
+        check(hipMemcpyAsync(a->red_host, d, 8, hipMemcpyDeviceToHost, a->stream), "copy");
+        a->sync();
+        *out = *a->red_host;
+    });
+}
+
+}  // extern "C"

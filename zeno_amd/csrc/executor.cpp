@@ -1,0 +1,271 @@
+// executor.cpp — the host side above the Backend boundary: compile() and
+// run_compiled() (src/compiler.rs:118-200, src/exec.rs:334-619,
+// src/noise.rs:91-192) driving the HIP backend only through the zb_* C ABI,
+// exactly as the reference's executor drives `dyn Backend`.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <unordered_map>
+
+#include "host.hpp"
+#include "zeno_hip.h"
+
+using namespace zeno;
+
+struct zr_compiled {
+    Compiled c;
+};
+
+namespace {
+
+struct BackendDeleter {
+    void operator()(zb_backend *h) const { zb_destroy(h); }
+};
+using BackendPtr = std::unique_ptr<zb_backend, BackendDeleter>;
+
+struct RunError {
+    int code;
+};
+
+void ok(int rc) {
+    if (rc < 0) throw RunError{rc};
+}
+
+const double kX[8] = {0, 0, 1, 0, 1, 0, 0, 0};
+const double kY[8] = {0, 0, 0, -1, 0, 1, 0, 0};
+const double kZ[4] = {1, 0, -1, 0};
+
+// noise.rs:91-100
+void apply_pauli(zb_backend *h, uint32_t q, uint64_t code) {
+    if (code == 1) ok(zb_apply_unitary(h, &q, 1, kX));
+    else if (code == 2) ok(zb_apply_unitary(h, &q, 1, kY));
+    else if (code == 3) ok(zb_apply_diagonal(h, &q, 1, kZ));
+}
+
+int measure(zb_backend *h, uint32_t q, double u) {
+    const int r = zb_measure(h, q, u);
+    ok(r);
+    return r;
+}
+
+// noise.rs:114-127: jump = collapse to 1 then X; no jump = the renormalised
+// diagonal diag(1, sqrt(1-g)) / sqrt(1 - P(jump)) in one sweep.
+void amplitude_damp(zb_backend *h, uint32_t q, double gamma, double u) {
+    const double p1 = zb_prob_one(h, q);
+    if (std::isnan(p1)) throw RunError{ZB_ERR_DEVICE};
+    const double pj = gamma * p1;
+    if (u < pj) {
+        measure(h, q, 0.0);
+        apply_pauli(h, q, 1);
+    } else {
+        const double s = 1.0 / std::sqrt(1.0 - pj);
+        const double d[4] = {s, 0.0, s * std::sqrt(1.0 - gamma), 0.0};
+        ok(zb_apply_diagonal(h, &q, 1, d));
+    }
+}
+
+// noise.rs:145-175, frozen draw order.
+void gate_noise(zb_backend *h, const zr_noise &m, const std::vector<uint32_t> &qs, Rng &rng) {
+    const int k = (int)qs.size();
+    const double pd = k == 1 ? m.depolarizing_1q : m.depolarizing_2q;
+    if (pd > 0.0 && rng.f64() < pd) {
+        const uint64_t idx = rng.range(1, 1ull << (2 * k));
+        for (int b = 0; b < k; ++b) apply_pauli(h, qs[b], (idx >> (2 * b)) & 3);
+    }
+    for (uint32_t q : qs) {
+        if (m.bit_flip > 0.0 && rng.f64() < m.bit_flip) apply_pauli(h, q, 1);
+        if (m.phase_flip > 0.0 && rng.f64() < m.phase_flip) apply_pauli(h, q, 3);
+        if (m.amplitude_damping > 0.0) {
+            const double u = rng.f64();
+            amplitude_damp(h, q, m.amplitude_damping, u);
+        }
+    }
+}
+
+// noise.rs:181-192
+int readout(const zr_noise &m, int bit, Rng &rng) {
+    const double p = bit ? m.readout_flip_1to0 : m.readout_flip_0to1;
+    if (p > 0.0 && rng.f64() < p) return !bit;
+    return bit;
+}
+
+void apply_op(zb_backend *h, const Op &o) {  // exec.rs:398-405
+    if (o.kind == kUnitary)
+        ok(zb_apply_unitary(h, o.qs.data(), (uint32_t)o.qs.size(), reinterpret_cast<const double *>(o.data.data())));
+    else if (o.kind == kDiagonal)
+        ok(zb_apply_diagonal(h, o.qs.data(), (uint32_t)o.qs.size(), reinterpret_cast<const double *>(o.data.data())));
+    else if (o.kind == kCx)
+        ok(zb_apply_cx(h, o.a, o.b));
+}
+
+// exec.rs:556-619
+void exec_shot_op(zb_backend *h, const Op &o, const zr_noise *nz, Rng &rng, uint64_t &clbits) {
+    if (o.is_if) {
+        const uint64_t mask = o.creg_len >= 64 ? ~0ull : ((1ull << o.creg_len) - 1);
+        if (((clbits >> o.creg_off) & mask) != o.value) return;
+    }
+    switch (o.kind) {
+        case kUnitary:
+        case kDiagonal:
+            apply_op(h, o);
+            if (nz) gate_noise(h, *nz, o.qs, rng);
+            break;
+        case kCx:
+            apply_op(h, o);
+            if (nz) gate_noise(h, *nz, {std::min(o.a, o.b), std::max(o.a, o.b)}, rng);
+            break;
+        case kMeasure: {
+            int bit = measure(h, o.a, rng.f64());
+            if (nz) bit = readout(*nz, bit, rng);
+            clbits = (clbits & ~(1ull << o.b)) | ((uint64_t)bit << o.b);
+            break;
+        }
+        case kReset:
+            if (measure(h, o.a, rng.f64())) ok(zb_apply_unitary(h, &o.a, 1, kX));
+            break;
+        default:
+            break;
+    }
+}
+
+bool trivial(const zr_noise *m) {
+    return !m || (m->depolarizing_1q == 0.0 && m->depolarizing_2q == 0.0 && m->bit_flip == 0.0 &&
+                  m->phase_flip == 0.0 && m->amplitude_damping == 0.0 && m->readout_flip_0to1 == 0.0 &&
+                  m->readout_flip_1to0 == 0.0);
+}
+
+bool valid(const zr_noise *m) {  // noise.rs:78-87
+    if (!m) return true;
+    for (double v : {m->depolarizing_1q, m->depolarizing_2q, m->bit_flip, m->phase_flip, m->amplitude_damping,
+                     m->readout_flip_0to1, m->readout_flip_1to0})
+        if (!(v >= 0.0 && v <= 1.0)) return false;
+    return true;
+}
+
+}  // namespace
+
+extern "C" {
+
+zr_compiled *zr_compile(const zr_instr *prog, int64_t n_instr, uint32_t n_qubits, const uint32_t *creg_sizes,
+                        uint32_t n_cregs, int fusion_max, int diag_max) {
+    try {
+        auto *c = new zr_compiled();
+        c->c = compile(prog, n_instr, n_qubits, creg_sizes, n_cregs, fusion_max, diag_max);
+        return c;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return nullptr;
+    }
+}
+
+void zr_compiled_free(zr_compiled *c) { delete c; }
+
+int64_t zr_compiled_nops(const zr_compiled *c) { return (int64_t)c->c.ops.size(); }
+
+int zr_compiled_op(const zr_compiled *c, int64_t i, int32_t *kind, int32_t *k, uint32_t *qs, double *data,
+                   int32_t *is_if, uint64_t *cond) {
+    if (i < 0 || i >= (int64_t)c->c.ops.size()) return ZB_ERR_ARG;
+    const Op &o = c->c.ops[(size_t)i];
+    *kind = o.kind;
+    *is_if = o.is_if;
+    cond[0] = o.creg_off, cond[1] = o.creg_len, cond[2] = o.value;
+    if (o.kind == kUnitary || o.kind == kDiagonal) {
+        *k = (int32_t)o.qs.size();
+        std::copy(o.qs.begin(), o.qs.end(), qs);
+        if (data) std::memcpy(data, o.data.data(), o.data.size() * sizeof(cplx));
+    } else {
+        *k = 2;
+        qs[0] = o.a, qs[1] = o.b;
+    }
+    return ZB_OK;
+}
+
+void zr_compiled_stats(const zr_compiled *c, int64_t *s) {
+    s[0] = c->c.stats.input_gates;
+    s[1] = c->c.stats.cancelled;
+    s[2] = c->c.stats.output_ops;
+    s[3] = c->c.stats.max_fused;
+    s[4] = c->c.dynamic;
+    s[5] = (int64_t)c->c.final_measures.size();
+}
+
+int64_t zr_compiled_final_measures(const zr_compiled *c, uint32_t *pairs) {
+    for (size_t i = 0; i < c->c.final_measures.size(); ++i)
+        pairs[2 * i] = c->c.final_measures[i].first, pairs[2 * i + 1] = c->c.final_measures[i].second;
+    return (int64_t)c->c.final_measures.size();
+}
+
+int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi) {
+    hi = std::min<int64_t>(hi, (int64_t)c->c.ops.size());
+    try {
+        for (int64_t i = std::max<int64_t>(lo, 0); i < hi; ++i) apply_op(h, c->c.ops[(size_t)i]);
+    } catch (const RunError &e) {
+        return e.code;
+    }
+    return ZB_OK;
+}
+
+int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t seed, const zr_noise *noise, int mode,
+               uint64_t *keys, uint64_t *counts, int64_t cap, double *sim_seconds) {
+    const Compiled &c = cc->c;
+    if (!valid(noise)) {
+        set_error("invalid noise model: every probability must be in [0, 1]");
+        return ZB_ERR_ARG;
+    }
+    const zr_noise *nz = trivial(noise) ? nullptr : noise;
+    std::unordered_map<uint64_t, uint64_t> raw;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        zb_backend *hp = nullptr;
+        ok(zb_create(c.n_qubits, precision, &hp));
+        BackendPtr h(hp);
+        ok(zb_set_mode(h.get(), mode));
+        if (!nz && !c.dynamic) {  // run_sampled (exec.rs:407-452)
+            for (const Op &o : c.ops) apply_op(h.get(), o);
+            ok(zb_finish(h.get()));
+            if (!c.final_measures.empty() && shots > 0) {
+                std::vector<uint64_t> smp(shots);
+                ok(zb_sample(h.get(), shots, splitmix64(seed), smp.data()));
+                std::unordered_map<uint64_t, uint64_t> basis;
+                for (uint64_t s : smp) ++basis[s];
+                for (const auto &kv : basis) {
+                    uint64_t cl = 0;
+                    for (const auto &fm : c.final_measures) {
+                        const uint64_t bit = (kv.first >> fm.first) & 1;
+                        cl = (cl & ~(1ull << fm.second)) | (bit << fm.second);
+                    }
+                    raw[cl] += kv.second;
+                }
+            }
+        } else if (c.dynamic || !c.final_measures.empty()) {  // run_per_shot (exec.rs:459-549)
+            for (uint64_t shot = 0; shot < shots; ++shot) {
+                ok(zb_reset_all(h.get()));
+                Rng rng(splitmix64(seed ^ shot));
+                uint64_t cl = 0;
+                for (const Op &o : c.ops) exec_shot_op(h.get(), o, nz, rng, cl);
+                for (const auto &fm : c.final_measures) {
+                    int bit = measure(h.get(), fm.first, rng.f64());
+                    if (nz) bit = readout(*nz, bit, rng);
+                    cl = (cl & ~(1ull << fm.second)) | ((uint64_t)bit << fm.second);
+                }
+                ++raw[cl];
+            }
+        }
+    } catch (const RunError &e) {
+        return e.code;
+    }
+    if (sim_seconds) *sim_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    std::vector<std::pair<uint64_t, uint64_t>> v(raw.begin(), raw.end());
+    std::sort(v.begin(), v.end());
+    if ((int64_t)v.size() > cap) {
+        set_error("key buffer too small");
+        return ZB_ERR_ARG;
+    }
+    for (size_t i = 0; i < v.size(); ++i) keys[i] = v[i].first, counts[i] = v[i].second;
+    return (int64_t)v.size();
+}
+
+}  // extern "C"

@@ -1,0 +1,364 @@
+"""Compiler and executor front door (src/compiler.rs, src/exec.rs, src/lib.rs).
+
+`compile` and `run_compiled` call the native host side (zr_*), which drives
+the HIP backend through the zb_* boundary.  Counts keys follow the qiskit
+convention of format_key (src/exec.rs:235-253).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+import os
+from typing import Optional
+
+import numpy as np
+
+from . import ir
+from ._lib import last_error, lib, ptr
+from .backend import HipBackend, MemoryBudget, ZenoError, _check
+
+
+class InvalidCircuit(ZenoError):
+    pass
+
+
+@dataclasses.dataclass
+class NoiseModel:
+    """noise::NoiseModel (src/noise.rs:37-46)."""
+
+    depolarizing_1q: float = 0.0
+    depolarizing_2q: float = 0.0
+    bit_flip: float = 0.0
+    phase_flip: float = 0.0
+    amplitude_damping: float = 0.0
+    readout_flip_0to1: float = 0.0
+    readout_flip_1to0: float = 0.0
+
+    def is_trivial(self) -> bool:
+        return all(v == 0.0 for v in dataclasses.astuple(self))
+
+    def validate(self):
+        for f in dataclasses.fields(self):
+            v = getattr(self, f.name)
+            if not (0.0 <= v <= 1.0):
+                raise ZenoError(f"invalid noise model: {f.name} = {v} is not a probability in [0, 1]")
+
+
+class _CNoise(C.Structure):
+    _fields_ = [(f.name, C.c_double) for f in dataclasses.fields(NoiseModel)]
+
+
+OP_U, OP_D, OP_CX, OP_MEAS, OP_RESET = 0, 1, 2, 3, 4
+
+
+class Compiled:
+    """compiler::Compiled (src/compiler.rs:80-94)."""
+
+    def __init__(self, prog: np.ndarray, n_qubits: int, cregs=None, fusion: int = 1, diag_max: int = 10):
+        prog = np.ascontiguousarray(prog, ir.INSTR_DTYPE)
+        cregs = np.asarray([n_qubits] if cregs is None else cregs, np.uint32)
+        self.n_qubits = n_qubits
+        self.cregs = [int(x) for x in cregs]
+        self._h = lib().zr_compile(ptr(prog), len(prog), n_qubits, ptr(cregs), len(cregs), fusion, diag_max)
+        if not self._h:
+            raise InvalidCircuit(last_error())
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().zr_compiled_free(self._h)
+            self._h = None
+
+    @property
+    def handle(self):
+        return self._h
+
+    def __len__(self):
+        return lib().zr_compiled_nops(self._h)
+
+    @property
+    def stats(self):
+        s = np.zeros(6, np.int64)
+        lib().zr_compiled_stats(self._h, ptr(s))
+        keys = ("input_gates", "cancelled", "output_ops", "max_fused", "dynamic", "n_final_measures")
+        return dict(zip(keys, (int(x) for x in s)))
+
+    @property
+    def dynamic(self) -> bool:
+        return bool(self.stats["dynamic"])
+
+    @property
+    def final_measures(self):
+        n = self.stats["n_final_measures"]
+        pairs = np.zeros(2 * max(n, 1), np.uint32)
+        lib().zr_compiled_final_measures(self._h, ptr(pairs))
+        return [(int(pairs[2 * i]), int(pairs[2 * i + 1])) for i in range(n)]
+
+    def op(self, i):
+        kind, k, is_if = C.c_int32(), C.c_int32(), C.c_int32()
+        qs = np.zeros(12, np.uint32)
+        data = np.zeros(2 * 4096, np.float64)
+        cond = np.zeros(3, np.uint64)
+        if lib().zr_compiled_op(self._h, i, C.byref(kind), C.byref(k), ptr(qs), ptr(data), C.byref(is_if), ptr(cond)) < 0:
+            raise IndexError(i)
+        kd, kk = kind.value, k.value
+        d = {"kind": kd, "qubits": [int(x) for x in qs[:kk]], "is_if": bool(is_if.value), "cond": [int(x) for x in cond]}
+        if kd == OP_U:
+            d["mat"] = data[: 2 * (1 << 2 * kk)].view(np.complex128).reshape(1 << kk, 1 << kk).copy()
+        elif kd == OP_D:
+            d["diag"] = data[: 2 * (1 << kk)].view(np.complex128).copy()
+        return d
+
+    def ops(self):
+        return [self.op(i) for i in range(len(self))]
+
+
+def compile(prog, n_qubits: int, cregs=None, fusion: int = 1, diag_max: int = 10) -> Compiled:
+    """compiler::compile (src/compiler.rs:118-200)."""
+    return Compiled(prog, n_qubits, cregs, fusion, diag_max)
+
+
+def format_key(clbits: int, cregs) -> str:
+    """exec.rs:235-253: cregs joined by spaces, last-declared leftmost."""
+    if not cregs:
+        return ""
+    parts, off = [], 0
+    for size in cregs:
+        mask = (1 << 64) - 1 if size >= 64 else (1 << size) - 1
+        parts.append(format((clbits >> off) & mask, f"0{size}b") if size else "")
+        off += size
+    return " ".join(reversed(parts))
+
+
+class Counts(dict):
+    """exec::Counts (src/exec.rs:77-112): bitstring -> count."""
+
+    def get(self, key, default=0):
+        return super().get(key, default)
+
+    def total(self) -> int:
+        return sum(self.values())
+
+    def probabilities(self):
+        t = max(self.total(), 1)
+        return sorted(((k, v / t) for k, v in self.items()), key=lambda kv: (-kv[1], kv[0]))
+
+
+@dataclasses.dataclass
+class RunResult:
+    """exec::RunResult (src/exec.rs:115-130)."""
+
+    counts: Counts
+    raw_counts: dict
+    shots: int
+    n_qubits: int
+    precision: int
+    backend: str
+    seed: int
+    sim_time: float
+    statevector: Optional[np.ndarray]
+    stats: dict
+    mem_bytes: int
+    notices: list
+
+
+def state_bytes(n_qubits: int, precision: int) -> int:
+    """mem::state_bytes (src/mem.rs:79-81): 8 B/amp in f32, 16 B in f64."""
+    return (1 << n_qubits) * (16 if precision == 64 else 8)
+
+
+def max_qubits(budget: int, precision: int) -> int:
+    """mem::max_qubits (src/mem.rs:84-93)."""
+    n = 0
+    while state_bytes(n + 1, precision) <= budget and n < 60:
+        n += 1
+    return n
+
+
+def device_budget_bytes(fraction: float = 0.9) -> int:
+    """Budget from HBM instead of RAM (`ZENO_MEM_BYTES` overrides, as in
+    src/mem.rs:11-17)."""
+    env = os.environ.get("ZENO_MEM_BYTES")
+    if env:
+        try:
+            return int(env)
+        except ValueError:
+            pass
+    try:
+        import torch
+
+        if torch.cuda.is_available():
+            free, _total = torch.cuda.mem_get_info()
+            return int(free * fraction)
+    except Exception:
+        pass
+    return 288 << 30
+
+
+def plan(n_qubits: int, precision: int, budget: int):
+    """mem::plan (src/mem.rs:114-130)."""
+    need = state_bytes(n_qubits, precision)
+    if need > budget:
+        raise MemoryBudget(
+            f"state vector for {n_qubits} qubits at f{precision} needs {need} B but the budget is {budget} B "
+            f"(max {max_qubits(budget, precision)} qubits at this precision)"
+        )
+    return need
+
+
+@dataclasses.dataclass
+class RunOptions:
+    """exec::RunOptions (src/exec.rs:29-72) for the HIP backend."""
+
+    shots: int = 1024
+    seed: Optional[int] = None
+    precision: Optional[int] = None  # None = auto: f64 if it fits, else f32
+    fusion_max: Optional[int] = None  # None = 1 (the CPU path's default)
+    mem_limit: Optional[int] = None
+    want_statevector: bool = False
+    noise: Optional[NoiseModel] = None
+    mode: int = HipBackend.TILED
+
+
+def resolve_precision(n_qubits: int, opts: RunOptions):
+    """exec.rs:261-304 with the HBM budget."""
+    budget = opts.mem_limit if opts.mem_limit is not None else device_budget_bytes()
+    notices = []
+    if opts.precision is not None:
+        plan(n_qubits, opts.precision, budget)
+        return opts.precision, notices
+    try:
+        plan(n_qubits, 64, budget)
+        return 64, notices
+    except MemoryBudget:
+        plan(n_qubits, 32, budget)
+        notices.append(f"auto-precision: {n_qubits} qubits exceed the f64 budget, using f32 (8 B/amplitude)")
+        return 32, notices
+
+
+def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
+    """exec::run_compiled (src/exec.rs:334-549) on the HIP backend."""
+    noise = opts.noise if (opts.noise is not None and not opts.noise.is_trivial()) else None
+    if noise is not None:
+        noise.validate()
+    prec, notices = resolve_precision(c.n_qubits, opts)
+    seed = opts.seed if opts.seed is not None else int.from_bytes(os.urandom(8), "little")
+    if noise is not None and opts.want_statevector:
+        raise InvalidCircuit(
+            "a noise model is active: the noisy state is a mixture over trajectories and has no single "
+            "final state vector (drop the statevector request or the noise model)"
+        )
+    if c.dynamic and opts.want_statevector:
+        raise InvalidCircuit("circuit is dynamic (mid-circuit measure/reset/if): it has no single final state vector")
+    statevector = None
+    name = "hip-f64" if prec == 64 else "hip-f32"
+    if opts.want_statevector:
+        import time
+
+        t0 = time.perf_counter()
+        with HipBackend(c.n_qubits, prec, opts.mode) as be:
+            be.apply_compiled(c)
+            be.finish()
+            statevector = be.statevector()
+            raw = {}
+            if c.final_measures and opts.shots > 0:
+                from collections import Counter
+
+                basis = Counter(be.sample(opts.shots, _splitmix64(seed)).tolist())
+                for b, n in basis.items():
+                    cl = 0
+                    for q, cb in c.final_measures:
+                        cl = (cl & ~(1 << cb)) | (((b >> q) & 1) << cb)
+                    raw[cl] = raw.get(cl, 0) + n
+        secs = time.perf_counter() - t0
+    else:
+        cap = max(opts.shots, 1)
+        keys = np.zeros(cap, np.uint64)
+        counts = np.zeros(cap, np.uint64)
+        secs_c = C.c_double(0.0)
+        cn = _CNoise(**dataclasses.asdict(noise)) if noise is not None else None
+        m = lib().zr_run(
+            c.handle, prec, opts.shots, seed, C.byref(cn) if cn is not None else None, opts.mode,
+            ptr(keys), ptr(counts), cap, C.byref(secs_c),
+        )
+        _check(m, "run")
+        raw = {int(k): int(v) for k, v in zip(keys[:m], counts[:m])}
+        secs = secs_c.value
+    if noise is not None:
+        notices.append("noise: trajectory sampling — per-shot execution, fusion disabled")
+    if not c.final_measures and not c.dynamic and opts.shots > 0:
+        notices.append("circuit has no measurements: counts are empty (use --statevector to inspect the state)")
+    counts = Counts()
+    for k, v in raw.items():
+        key = format_key(k, c.cregs)
+        counts[key] = counts.get(key) + v
+    return RunResult(
+        counts=counts, raw_counts=raw, shots=opts.shots, n_qubits=c.n_qubits, precision=prec, backend=name,
+        seed=seed, sim_time=secs, statevector=statevector, stats=c.stats,
+        mem_bytes=state_bytes(c.n_qubits, prec), notices=notices,
+    )
+
+
+def _splitmix64(x: int) -> int:
+    M = (1 << 64) - 1
+    z = (x + 0x9E3779B97F4A7C15) & M
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M
+    return z ^ (z >> 31)
+
+
+def compile_options(opts: RunOptions) -> int:
+    """lib.rs:224-240: noisy runs compile with fusion 0; the default dense
+    fusion width is 1 (the reference CPU default)."""
+    if opts.noise is not None and not opts.noise.is_trivial():
+        return 0
+    return 1 if opts.fusion_max is None else opts.fusion_max
+
+
+def run_program(prog, n_qubits: int, cregs=None, opts: Optional[RunOptions] = None) -> RunResult:
+    """lib.rs:242-250 run_program."""
+    opts = opts or RunOptions()
+    return run_compiled(compile(prog, n_qubits, cregs, fusion=compile_options(opts)), opts)
+
+
+class Simulator:
+    """Builder-style front door (src/lib.rs:139-222)."""
+
+    def __init__(self):
+        self.opts = RunOptions()
+
+    def shots(self, s):
+        self.opts.shots = s
+        return self
+
+    def seed(self, s):
+        self.opts.seed = s
+        return self
+
+    def precision(self, p):
+        self.opts.precision = p
+        return self
+
+    def fusion(self, k):
+        self.opts.fusion_max = k
+        return self
+
+    def mem_limit(self, b):
+        self.opts.mem_limit = b
+        return self
+
+    def noise(self, m: NoiseModel):
+        self.opts.noise = m
+        return self
+
+    def mode(self, m):
+        self.opts.mode = m
+        return self
+
+    def run(self, circuit) -> RunResult:
+        prog, n, cregs = circuit.to_program()
+        return run_program(prog, n, cregs, self.opts)
+
+    def statevector(self, circuit) -> np.ndarray:
+        o = dataclasses.replace(self.opts, want_statevector=True, shots=0)
+        prog, n, cregs = circuit.to_program()
+        return run_program(prog, n, cregs, o).statevector
