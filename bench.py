@@ -31,51 +31,48 @@ def _log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def op_class(op, n_sorted=None):
-    """Kernel class of a compiled op and its algorithmic bytes factor (in
-    units of 2^n * s): sweeps 4, CX 2."""
-    k = len(op["qubits"])
-    if op["kind"] == 0:
-        return ("k_1q" if k == 1 else f"k_kq<{k}>"), 4
-    if op["kind"] == 1:
-        return ("k_diag1" if k == 1 else "k_diagk"), 4
-    return "k_cx", 2
+def effective_bytes(ops, n, s):
+    """Bytes if every compiled op were its own sweep (4*2^n*s, CX 2*2^n*s),
+    plus the zero init: the op stream's size, not the passes executed."""
+    unit = (1 << n) * s
+    return 2 * unit + sum((2 if o["kind"] == 2 else 4) * unit for o in ops)
 
 
-def run_gpu(args, comp, ops, prec, torch):
+TUNE = {}
+
+
+def run_gpu(n, comp, prec, mode, steps, warmup, torch):
+    """Time `steps` full circuits (zero init + every op + finish) after
+    `warmup` untimed ones; kernel durations come from HIP events the library
+    records around each launch on its own stream."""
     import zeno_amd as z
 
-    n = args.n
-    s = 8 if prec == 64 else 4
-    unit = (1 << n) * s
-    be = z.HipBackend(n, prec, args.mode)
-    ext = torch.cuda.ExternalStream(be.stream())
-    classes = [op_class(o) for o in ops]
-    per_class = {}
-    step_times = []
-    total_bytes = 2 * unit + sum(f * unit for _, f in classes)  # zero init + sweeps
-    for step in range(args.warmup + args.steps):
-        timed = step >= args.warmup
-        evs = [torch.cuda.Event(enable_timing=True) for _ in range(len(ops) + 1)]
+    be = z.HipBackend(n, prec, mode)
+    for k, v in TUNE.items():
+        be.tune(k, v)
+    step_times, recs = [], []
+    for step in range(warmup + steps):
+        timed = step >= warmup
         torch.cuda.synchronize()
+        if timed:
+            be.profile_begin()
         t0 = time.perf_counter()
         be.reset_all()
-        for i in range(len(ops)):
-            evs[i].record(ext)
-            be.apply_compiled(comp, i, i + 1)
-        evs[-1].record(ext)
+        be.apply_compiled(comp)
         be.finish()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         if timed:
             step_times.append(dt)
-            for i, (cls, f) in enumerate(classes):
-                ms = evs[i].elapsed_time(evs[i + 1])
-                c = per_class.setdefault(cls, {"launches": 0, "ms": 0.0, "bytes_per_launch": f * unit})
-                c["launches"] += 1
-                c["ms"] += ms
-        _log(f"  f{prec} step {step} {'timed' if timed else 'warmup'}: {dt:.3f} s")
-    return be, step_times, per_class, total_bytes
+            recs += be.profile_end()
+        _log(f"  n={n} f{prec} mode={mode} step {step} {'timed' if timed else 'warmup'}: {dt:.4f} s")
+    per_class = {}
+    for name, ms, nbytes in recs:
+        c = per_class.setdefault(name, {"launches": 0, "ms": 0.0, "bytes_per_launch": nbytes})
+        c["launches"] += 1
+        c["ms"] += ms
+    circuit_bytes = sum(b for _, _, b in recs) / max(1, steps)
+    return be, sum(step_times) / len(step_times), per_class, circuit_bytes
 
 
 def cpu_baseline(args, prog):
@@ -122,8 +119,13 @@ def main():
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", type=int, default=1, help="0 exact per-op sweeps, 1 tiled")
     ap.add_argument("--cpu-ops", type=int, default=8)
+    ap.add_argument("--also", default="", help="extra fusion:mode pairs to time once, e.g. 5:0,5:1")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tune", default="", help="tiled-kernel knobs, e.g. prefetch=1,win_grid=1024")
     args = ap.parse_args()
+    for kv in [x for x in args.tune.split(",") if x]:
+        k, v = kv.split("=")
+        TUNE[k] = int(v)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(
             "bench.py measures one MI355X: the state-vector path is benchmarked on a single GPU; "
@@ -148,15 +150,25 @@ def main():
     ops = comp.ops()
     _log(f"compiled {comp.stats['input_gates']} gates -> {len(ops)} ops at fusion {args.fusion} in {t_compile:.2f} s")
 
-    be64, t64, cls64, bytes64 = run_gpu(args, comp, ops, 64, torch)
-    be32, t32, cls32, bytes32 = run_gpu(args, comp, ops, 32, torch)
+    be64, s64, cls64, bytes64 = run_gpu(n, comp, 64, args.mode, args.steps, args.warmup, torch)
+    be32, s32, cls32, bytes32 = run_gpu(n, comp, 32, args.mode, args.steps, args.warmup, torch)
     err = be32.max_abs_diff(be64)
     norm64 = be64.norm_sqr()
     be32.close()
     be64.close()
 
-    s64 = sum(t64) / len(t64)
-    s32 = sum(t32) / len(t32)
+    alternatives = []
+    for spec in [x for x in args.also.split(",") if x]:
+        fk, md = (int(v) for v in spec.split(":"))
+        c2 = z.compile(prog, n, fusion=fk)
+        for prec in (64, 32):
+            b2, t2, k2, by2 = run_gpu(n, c2, prec, md, 1, 1, torch)
+            b2.close()
+            alternatives.append({"fusion": fk, "mode": "tiled" if md == 1 else "exact", "precision": f"f{prec}",
+                                 "ops": len(c2), "s_per_circuit": round(t2, 4), "hbm_gbps": round(by2 / t2 / 1e9, 2),
+                                 "kernels": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)}
+                                             for k, v in k2.items()}})
+
     dom_cls, dom = max(cls64.items(), key=lambda kv: kv[1]["ms"])
     avg_ms = dom["ms"] / dom["launches"]
     achieved = dom["bytes_per_launch"] / (avg_ms * 1e-3) / 1e9
@@ -195,8 +207,10 @@ def main():
             "depth": args.depth,
             "fusion": args.fusion,
             "compiled_ops": len(ops),
+            "tune": dict(TUNE),
             "input_gates": comp.stats["input_gates"],
         },
+        "effective_gbps_f64": round(effective_bytes(ops, n, 8) / s64 / 1e9, 2),
         "s_per_circuit_f64": round(s64, 4),
         "s_per_circuit_f32": round(s32, 4),
         "hbm_gbps_f32": round(bytes32 / s32 / 1e9, 2),
@@ -204,6 +218,8 @@ def main():
         "f32_max_abs_damp_vs_f64": err,
         "f64_norm_sqr": norm64,
         "kernels_f64": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls64.items()},
+        "kernels_f32": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls32.items()},
+        "alternatives": alternatives,
         "roofline": {
             "bound": "hbm",
             "kernel": dom_cls,

@@ -135,6 +135,26 @@ void *zb_stream(zb_backend *h);
 /* max_i |a_i - b_i| over two states of equal size (any precisions), f64,
  * computed on the device; used to state the f32 error against f64. */
 int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out);
+/* Tuning knobs of the tiled kernel (flushes first): "prefetch" (0/1:
+ * issue the next tile's HBM loads into registers while the current tile
+ * computes) and "win_grid" (workgroups per window launch). */
+int zb_tune(zb_backend *h, const char *key, int64_t value);
+/* Launch profiling.  zb_profile_begin flushes, synchronises and starts
+ * recording a HIP event pair around every state-sweep kernel launch on the
+ * backend's stream.  zb_profile_end flushes, synchronises, stops, and
+ * writes per launch: the kernel id (ZB_K_*), its duration in ms and its
+ * algorithmic HBM bytes (4*2^n*s per sweep, 2*2^n*s for CX and zero init).
+ * Returns the number of records written (<= cap) or < 0. */
+#define ZB_K_ZERO 0
+#define ZB_K_1Q 1
+#define ZB_K_CX 2
+#define ZB_K_DIAG1 3
+#define ZB_K_DIAGK 4
+#define ZB_K_WINDOW 5
+#define ZB_K_KQ 10 /* + k for the dense k-qubit kernel */
+int zb_profile_begin(zb_backend *h);
+int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms,
+                       double *bytes, int64_t cap);
 
 /* ---------------------------------------------------------------------
  * Layer 2: compiler + executor (the host side above the boundary)

@@ -89,30 +89,36 @@ def test_single_ops_bit_exact(orc, prec):
     g.close()
 
 
+MODES = [z.HipBackend.EXACT, z.HipBackend.TILED]
+
+
+@pytest.mark.parametrize("mode", MODES)
 @pytest.mark.parametrize("prec", [64, 32])
-def test_random_circuits_bit_exact(orc, prec):
-    for n in (2, 3, 5, 8, 11, 14, 17, 20):
+def test_random_circuits_bit_exact(orc, prec, mode):
+    for n in (2, 3, 5, 8, 11, 12, 13, 14, 17, 20):
         prog = orc.gen_random(n, 12, 1000 + n)
         for fusion in (0, 1, 2, 4, 6):
-            _say(f"random_circuit n={n} fusion={fusion} prec={prec}")
+            _say(f"random_circuit n={n} fusion={fusion} prec={prec} mode={mode}")
             comp = orc.Compiled(prog, n, fusion=fusion)
             o = orc.State(n, prec)
             o.apply_compiled(comp)
-            with z.HipBackend(n, prec, z.HipBackend.EXACT) as g:
+            with z.HipBackend(n, prec, mode) as g:
                 g.apply_compiled(z.compile(prog, n, fusion=fusion))
                 _assert_same(g, o, f"n={n} fusion={fusion}")
 
 
-def test_adversarial_circuits_bit_exact(orc):
-    cases = [(random_mixed(orc, n, 80, 9000 + n), n) for n in (3, 6, 10, 13)]
+@pytest.mark.parametrize("mode", MODES)
+def test_adversarial_circuits_bit_exact(orc, mode):
+    cases = [(random_mixed(orc, n, 80, 9000 + n), n) for n in (3, 6, 10, 13, 15)]
     cases += [(diag_ladder(orc), 6), (disjoint_blocks(orc), 8), (orc.gen_u3_qft(12, 5), 12),
-              (orc.gen_matching(16, 10, 3, 5), 16)]
+              (orc.gen_u3_qft(17, 5), 17), (orc.gen_matching(16, 10, 3, 5), 16),
+              (orc.gen_matching(19, 12, 4, 3), 19)]
     for prog, n in cases:
         for fusion in (0, 1, 3, 5):
-            _say(f"adversarial n={n} fusion={fusion}")
+            _say(f"adversarial n={n} fusion={fusion} mode={mode}")
             o = orc.State(n, 64)
             o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
-            with z.HipBackend(n, 64, z.HipBackend.EXACT) as g:
+            with z.HipBackend(n, 64, mode) as g:
                 g.apply_compiled(z.compile(prog, n, fusion=fusion))
                 _assert_same(g, o, f"n={n} fusion={fusion}")
 

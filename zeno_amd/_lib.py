@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libzeno_hip.so")
 CSRC = os.path.join(_HERE, "csrc")
 
 ZB_OK, ZB_ERR_ARG, ZB_ERR_UNSUPPORTED, ZB_ERR_MEMORY, ZB_ERR_DEVICE = 0, -1, -2, -3, -4
+KERNEL_NAMES = {0: "k_zero", 1: "k_1q", 2: "k_cx", 3: "k_diag1", 4: "k_diagk", 5: "k_window",
+                **{10 + k: f"k_kq<{k}>" for k in range(2, 7)}}
 
 # Every exported symbol of include/zeno_hip.h with its ctypes signature.
 P, U64, I64, I32, U32, D = C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_uint32, C.c_double
@@ -41,6 +43,9 @@ SIGNATURES = {
     "zb_set_mode": (I32, [P, I32]),
     "zb_stream": (P, [P]),
     "zb_max_abs_diff": (I32, [P, P, P]),
+    "zb_profile_begin": (I32, [P]),
+    "zb_tune": (I32, [P, C.c_char_p, I64]),
+    "zb_profile_end": (I64, [P, P, P, P, I64]),
     "zr_compile": (P, [P, I64, U32, P, U32, I32, I32]),
     "zr_compiled_free": (None, [P]),
     "zr_compiled_nops": (I64, [P]),

@@ -158,6 +158,23 @@ class HipBackend:
         _check(lib().zb_max_abs_diff(self._h, other._h, C.byref(out)), "max_abs_diff")
         return out.value
 
+    def tune(self, key: str, value: int):
+        """Set a tiled-kernel knob ("prefetch", "win_grid")."""
+        _check(lib().zb_tune(self._h, key.encode(), int(value)), "tune")
+
+    def profile_begin(self):
+        """Record HIP events around every state-sweep launch from now on."""
+        _check(lib().zb_profile_begin(self._h), "profile_begin")
+
+    def profile_end(self, cap: int = 1 << 16):
+        """[(kernel name, ms, algorithmic bytes)] for the launches since
+        profile_begin."""
+        k = np.zeros(cap, np.int32)
+        ms = np.zeros(cap, np.float64)
+        by = np.zeros(cap, np.float64)
+        n = _check(lib().zb_profile_end(self._h, ptr(k), ptr(ms), ptr(by), cap), "profile_end")
+        return [(_lib.KERNEL_NAMES.get(int(k[i]), str(int(k[i]))), float(ms[i]), float(by[i])) for i in range(n)]
+
     def apply_compiled(self, compiled, lo: int = 0, hi: int | None = None):
         hi = len(compiled) if hi is None else hi
         _check(lib().zr_apply_ops(self._h, compiled.handle, lo, hi), "apply_ops")

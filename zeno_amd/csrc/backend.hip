@@ -48,6 +48,8 @@ struct zb_backend {
     void *re = nullptr, *im = nullptr;
     hipStream_t stream = nullptr;
     int mode = 1;
+    bool prefetch = false;  // tiled kernel variant: register prefetch of the next tile
+    unsigned win_grid = 2048;
     std::string name;
     // parameter arena
     char *arena_dev = nullptr, *arena_host = nullptr;
@@ -59,6 +61,44 @@ struct zb_backend {
     // grow-only scratch for the sampler
     char *scratch = nullptr;
     size_t scratch_cap = 0;
+    // pending tiled window (mode 1): ops with global qubits, data in T
+    struct POp {
+        uint32_t kind, k;
+        uint32_t q[dev::kMaxDiagK];
+        uint32_t data_off;
+    };
+    uint64_t win_mask = 0;
+    std::vector<POp> win_ops;
+    std::vector<unsigned char> win_data;
+    // launch profiling: HIP events around every state-sweep launch
+    struct Rec {
+        int kernel;
+        double bytes;
+        hipEvent_t a, b;
+    };
+    bool prof = false;
+    std::vector<Rec> recs;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_next = 0;
+    hipEvent_t next_event() {
+        if (ev_next == ev_pool.size()) {
+            hipEvent_t e;
+            check(hipEventCreate(&e), "hipEventCreate");
+            ev_pool.push_back(e);
+        }
+        return ev_pool[ev_next++];
+    }
+    // Brackets one launch with events when profiling (kernel ids in zeno_hip.h).
+    template <class F>
+    void timed(int kernel, double bytes, F &&launch) {
+        if (!prof) return launch();
+        Rec r{kernel, bytes, next_event(), next_event()};
+        check(hipEventRecord(r.a, stream), "hipEventRecord");
+        launch();
+        check(hipEventRecord(r.b, stream), "hipEventRecord");
+        recs.push_back(r);
+    }
+    double sweep_bytes() const { return 4.0 * (double)len * (double)elem; }
 
     void sync() { check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
 
@@ -107,10 +147,14 @@ struct zb_backend {
 
 namespace {
 
+void flush(zb_backend *h);
+
 template <typename T>
 void launch_zero(zb_backend *h) {
-    hipLaunchKernelGGL(dev::k_zero<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
-                       h->len);
+    h->timed(ZB_K_ZERO, h->sweep_bytes() / 2, [&] {
+        hipLaunchKernelGGL(dev::k_zero<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), h->len);
+    });
     check(hipGetLastError(), "k_zero");
 }
 
@@ -127,13 +171,15 @@ void to_T(const double *src, size_t n, std::vector<T> &re, std::vector<T> &im) {
 template <typename T, int K>
 void launch_kq(zb_backend *h, const T *dmat, const dev::Qubits &qs) {
     const uint64_t groups = h->len >> K;
-    hipLaunchKernelGGL((dev::k_kq<T, K>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), dmat, qs, groups);
+    h->timed(ZB_K_KQ + K, h->sweep_bytes(), [&] {
+        hipLaunchKernelGGL((dev::k_kq<T, K>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), dmat, qs, groups);
+    });
     check(hipGetLastError(), "k_kq");
 }
 
 template <typename T>
-void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
+void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
     const size_t dim = (size_t)1 << k;
     std::vector<T> mr, mi;
     to_T<T>(mat, dim * dim, mr, mi);
@@ -141,8 +187,10 @@ void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
         dev::Mat2<T> m;
         for (int i = 0; i < 4; ++i) m.r[i] = mr[i], m.i[i] = mi[i];
         const uint64_t half = h->len >> 1;
-        hipLaunchKernelGGL(dev::k_1q<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
-                           qs[0], half, m);
+        h->timed(ZB_K_1Q, h->sweep_bytes(), [&] {
+            hipLaunchKernelGGL(dev::k_1q<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                               h->I<T>(), qs[0], half, m);
+        });
         check(hipGetLastError(), "k_1q");
         return;
     }
@@ -163,14 +211,16 @@ void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
 }
 
 template <typename T>
-void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
+void exact_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
     const size_t dim = (size_t)1 << k;
     std::vector<T> dr, di;
     to_T<T>(d, dim, dr, di);
     if (k == 1) {
         const uint64_t half = h->len >> 1;
-        hipLaunchKernelGGL(dev::k_diag1<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                           h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
+        h->timed(ZB_K_DIAG1, h->sweep_bytes(), [&] {
+            hipLaunchKernelGGL(dev::k_diag1<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                               h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
+        });
         check(hipGetLastError(), "k_diag1");
         return;
     }
@@ -181,22 +231,225 @@ void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     dev::Qubits q{};
     q.k = (int)k;
     for (uint32_t b = 0; b < k; ++b) q.q[b] = qs[b];
-    hipLaunchKernelGGL(dev::k_diagk<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), tab, q, h->len);
+    h->timed(ZB_K_DIAGK, h->sweep_bytes(), [&] {
+        hipLaunchKernelGGL(dev::k_diagk<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), tab, q, h->len);
+    });
     check(hipGetLastError(), "k_diagk");
 }
 
 template <typename T>
-void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
+void exact_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
     const uint64_t groups = h->len >> 2;
-    hipLaunchKernelGGL(dev::k_cx<T>, dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(), h->I<T>(),
-                       std::min(c, t), std::max(c, t), 1ull << c, 1ull << t, groups);
+    h->timed(ZB_K_CX, h->sweep_bytes() / 2, [&] {
+        hipLaunchKernelGGL(dev::k_cx<T>, dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), std::min(c, t), std::max(c, t), 1ull << c, 1ull << t, groups);
+    });
     check(hipGetLastError(), "k_cx");
+}
+
+// ---------------------------------------------------------------- windows
+// Tiled mode: ops are queued into the open window while the window's qubit
+// set Q (the lowest `low` qubits plus every qubit an op needs inside the
+// tile) stays within the tile width; otherwise the window is flushed as one
+// k_window pass.  A window holding a single op runs the per-op kernel, and
+// so do dense ops on 5 or 6 qubits and every op of a state under 8 qubits.
+uint32_t tile_bits(const zb_backend *h) { return std::min<uint32_t>(h->n, dev::kTileBits); }
+bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= 8; }
+
+uint64_t low_mask(const zb_backend *h) {
+    const uint32_t low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, h->n);
+    return (1ull << low) - 1;
+}
+
+template <typename T>
+void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint64_t need, const double *src,
+             size_t n_complex) {
+    const uint64_t m = tile_bits(h);
+    if (!h->win_ops.empty() && (uint64_t)__builtin_popcountll(h->win_mask | need) > m) flush(h);
+    if (h->win_ops.empty()) h->win_mask = low_mask(h);
+    h->win_mask |= need;
+    zb_backend::POp op{};
+    op.kind = kind;
+    op.k = k;
+    for (uint32_t b = 0; b < k; ++b) op.q[b] = qs[b];
+    op.data_off = (uint32_t)(h->win_data.size() / sizeof(T));
+    const size_t at = h->win_data.size();
+    h->win_data.resize(at + 2 * n_complex * sizeof(T));
+    T *dst = reinterpret_cast<T *>(h->win_data.data() + at);
+    for (size_t i = 0; i < n_complex; ++i) {
+        dst[i] = (T)src[2 * i];  // T::fr
+        dst[n_complex + i] = (T)src[2 * i + 1];
+    }
+    h->win_ops.push_back(op);
+}
+
+template <typename T>
+void flush_T(zb_backend *h) {
+    if (h->win_ops.size() == 1) {  // one op: the per-op kernel is the cheaper pass
+        const zb_backend::POp op = h->win_ops[0];
+        const T *d = reinterpret_cast<const T *>(h->win_data.data()) + op.data_off;
+        const size_t len = op.kind == dev::kWDiag ? (1u << op.k) : (1u << (2 * op.k));
+        std::vector<double> inter(2 * len);
+        for (size_t i = 0; i < len; ++i) inter[2 * i] = (double)d[i], inter[2 * i + 1] = (double)d[len + i];
+        h->win_ops.clear();
+        h->win_data.clear();
+        if (op.kind == dev::kWCx) exact_cx_T<T>(h, op.q[1], op.q[0]);
+        else if (op.kind == dev::kWDiag) exact_diag_T<T>(h, op.q, op.k, inter.data());
+        else exact_unitary_T<T>(h, op.q, op.k, inter.data());
+        return;
+    }
+    const uint32_t m = tile_bits(h);
+    uint64_t mask = h->win_mask;
+    for (uint32_t q = 0; q < h->n && (uint32_t)__builtin_popcountll(mask) < m; ++q) mask |= 1ull << q;
+    dev::WinHeader hdr{};
+    hdr.m = m;
+    int local[64];
+    for (int i = 0; i < 64; ++i) local[i] = -1;
+    uint32_t idx = 0;
+    for (uint32_t q = 0; q < h->n; ++q)
+        if ((mask >> q) & 1) hdr.q[idx] = q, local[q] = (int)idx++;
+    // Stages: greedy over the ops in order; a stage's slots must hold every
+    // local bit its ops act on (CX: the target only; diagonals: none).
+    auto needs = [&](const zb_backend::POp &p) {
+        uint32_t nb = 0;
+        if (p.kind == dev::kW1q || p.kind == dev::kWDense)
+            for (uint32_t b = 0; b < p.k; ++b) nb |= 1u << local[p.q[b]];
+        else if (p.kind == dev::kWCx)
+            nb |= 1u << local[p.q[0]];
+        return nb;
+    };
+    std::vector<dev::WinStage> stages;
+    std::vector<uint32_t> stage_slots;  // bitmask of local bits per stage
+    uint32_t cur = 0;
+    size_t begin = 0;
+    bool dense = false;
+    auto close = [&](size_t end) {
+        uint32_t sl = cur;
+        for (uint32_t b = 0; b < m && __builtin_popcount(sl) < dev::kSlots; ++b) sl |= 1u << b;
+        dev::WinStage st{};
+        int i = 0;
+        for (uint32_t b = 0; b < m; ++b)
+            if ((sl >> b) & 1) st.slot[i++] = b;
+        st.op_begin = (uint32_t)begin;
+        st.op_end = (uint32_t)end;
+        stages.push_back(st);
+        stage_slots.push_back(sl);
+    };
+    for (size_t i = 0; i < h->win_ops.size(); ++i) {
+        const uint32_t nb = needs(h->win_ops[i]);
+        if (__builtin_popcount(cur | nb) > dev::kSlots) {
+            close(i);
+            begin = i;
+            cur = 0;
+        }
+        cur |= nb;
+    }
+    close(h->win_ops.size());
+    hdr.nstages = (uint32_t)stages.size();
+    std::vector<dev::WinOp> wops(h->win_ops.size());
+    for (size_t si = 0; si < stages.size(); ++si) {
+        const uint32_t sl = stage_slots[si];
+        auto slot_of = [&](int lb) {
+            int i = 0;
+            for (int b = 0; b < lb; ++b) i += (sl >> b) & 1;
+            return (uint32_t)i;
+        };
+        for (uint32_t oi = stages[si].op_begin; oi < stages[si].op_end; ++oi) {
+            const zb_backend::POp &p = h->win_ops[oi];
+            dev::WinOp &w = wops[oi];
+            w = dev::WinOp{};
+            w.kind = p.kind;
+            w.k = p.k;
+            w.data_off = p.data_off;
+            if (p.kind == dev::kW1q) {
+                w.sel = slot_of(local[p.q[0]]);
+            } else if (p.kind == dev::kWDense) {
+                for (uint32_t b = 0; b < p.k; ++b) w.sel |= 1u << slot_of(local[p.q[b]]);
+                dense = true;
+            } else if (p.kind == dev::kWCx) {
+                const uint32_t tq = p.q[0], cq = p.q[1];
+                w.sel = slot_of(local[tq]);
+                if (local[cq] >= 0 && ((sl >> local[cq]) & 1)) {
+                    w.sel |= slot_of(local[cq]) << 2 | dev::kCtrlSlot << 4;
+                } else if (local[cq] >= 0) {
+                    w.sel |= dev::kCtrlLocal << 4;
+                    w.aux = (uint32_t)local[cq];
+                } else {
+                    w.sel |= dev::kCtrlFixed << 4;
+                    w.aux = cq;
+                }
+            } else {
+                for (uint32_t b = 0; b < p.k; ++b)
+                    w.bits[b] = local[p.q[b]] >= 0 ? (uint32_t)local[p.q[b]] : (dev::kFixedBit | p.q[b]);
+            }
+        }
+    }
+    auto pad = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t hb = pad(sizeof(hdr)), sb = pad(stages.size() * sizeof(dev::WinStage)),
+                 ob = pad(wops.size() * sizeof(dev::WinOp));
+    std::vector<unsigned char> blob(hb + sb + ob + h->win_data.size());
+    std::memcpy(blob.data(), &hdr, sizeof(hdr));
+    std::memcpy(blob.data() + hb, stages.data(), stages.size() * sizeof(dev::WinStage));
+    std::memcpy(blob.data() + hb + sb, wops.data(), wops.size() * sizeof(dev::WinOp));
+    std::memcpy(blob.data() + hb + sb + ob, h->win_data.data(), h->win_data.size());
+    unsigned char *d = static_cast<unsigned char *>(h->upload(blob.data(), blob.size()));
+    const uint64_t tiles = h->len >> m;
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, h->win_grid);
+    const auto *dh = reinterpret_cast<const dev::WinHeader *>(d);
+    const auto *ds = reinterpret_cast<const dev::WinStage *>(d + hb);
+    const auto *dop = reinterpret_cast<const dev::WinOp *>(d + hb + sb);
+    const auto *dd = reinterpret_cast<const T *>(d + hb + sb + ob);
+    h->timed(ZB_K_WINDOW, h->sweep_bytes(), [&] {
+        const dim3 g(grid), b(dev::kBlock);
+        if (dense && h->prefetch)
+            hipLaunchKernelGGL((dev::k_window<T, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+        else if (dense)
+            hipLaunchKernelGGL((dev::k_window<T, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+        else if (h->prefetch)
+            hipLaunchKernelGGL((dev::k_window<T, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+        else
+            hipLaunchKernelGGL((dev::k_window<T, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+    });
+    check(hipGetLastError(), "k_window");
+    h->win_ops.clear();
+    h->win_data.clear();
+}
+
+void flush(zb_backend *h) {
+    if (h->win_ops.empty()) return;
+    if (h->prec == 64) flush_T<double>(h);
+    else flush_T<float>(h);
+}
+
+template <typename T>
+void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
+    if (!tiled(h) || k > (uint32_t)dev::kSlots) {
+        flush(h);
+        return exact_unitary_T<T>(h, qs, k, mat);
+    }
+    uint64_t need = 0;
+    for (uint32_t b = 0; b < k; ++b) need |= 1ull << qs[b];
+    enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, qs, k, need, mat, (size_t)1 << (2 * k));
+}
+
+template <typename T>
+void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
+    if (!tiled(h)) return exact_diag_T<T>(h, qs, k, d);
+    enqueue<T>(h, dev::kWDiag, qs, k, 0, d, (size_t)1 << k);
+}
+
+template <typename T>
+void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
+    if (!tiled(h)) return exact_cx_T<T>(h, c, t);
+    const uint32_t qs[2] = {t, c};  // bits[0] = target, bits[1] = control
+    enqueue<T>(h, dev::kWCx, qs, 2, 1ull << t, nullptr, 0);
 }
 
 // Fixed-order reduction (mode 0: norm, mode 1: prob_one of q).
 template <typename T>
 double reduce_T(zb_backend *h, int mode, uint32_t q) {
+    flush(h);
     uint64_t n = mode ? h->len >> 1 : h->len;
     uint64_t nb = (n + dev::kReduceSpan - 1) / dev::kReduceSpan;
     h->ensure_red(2 * nb + 2);
@@ -218,6 +471,7 @@ double reduce_T(zb_backend *h, int mode, uint32_t q) {
 
 template <typename T>
 int measure_T(zb_backend *h, uint32_t q, double u) {
+    flush(h);
     const double p1 = reduce_T<T>(h, 1, q);
     const int outcome = u < p1;
     const double p = outcome ? p1 : 1.0 - p1;
@@ -241,6 +495,7 @@ void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
 // (passes 2-4) run on the host, in the reference's order.
 template <typename T>
 void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
+    flush(h);
     if (shots == 0) return;
     const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
     const uint64_t nch = (h->len + chunk - 1) / chunk;
@@ -372,6 +627,7 @@ void zb_destroy(zb_backend *h) {
     if (h->red_dev) (void)hipFree(h->red_dev);
     if (h->red_host) (void)hipHostFree(h->red_host);
     if (h->scratch) (void)hipFree(h->scratch);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
 }
@@ -386,6 +642,8 @@ int zb_set_mode(zb_backend *h, int mode) {
         set_error("mode must be 0 (exact) or 1 (tiled)");
         return ZB_ERR_ARG;
     }
+    const int rc = guarded([&] { flush(h); });
+    if (rc != ZB_OK) return rc;
     h->mode = mode;
     return ZB_OK;
 }
@@ -445,6 +703,8 @@ int zb_measure(zb_backend *h, uint32_t q, double u) {
 
 int zb_reset_all(zb_backend *h) {
     return guarded([&] {
+        h->win_ops.clear();  // pending ops act on a state that is being overwritten
+        h->win_data.clear();
         if (h->prec == 64) launch_zero<double>(h);
         else launch_zero<float>(h);
         h->sync();
@@ -460,6 +720,7 @@ int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
 
 int zb_chunk_sums(zb_backend *h, double *out) {
     return guarded([&] {
+        flush(h);
         const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
         const uint64_t nch = (h->len + chunk - 1) / chunk;
         double *d = (double *)h->scratch_get(nch * 8);
@@ -472,6 +733,7 @@ int zb_chunk_sums(zb_backend *h, double *out) {
 
 int zb_state_raw(zb_backend *h, void *re, void *im) {
     return guarded([&] {
+        flush(h);
         check(hipMemcpyAsync(re, h->re, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy re");
         check(hipMemcpyAsync(im, h->im, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy im");
         h->sync();
@@ -480,6 +742,8 @@ int zb_state_raw(zb_backend *h, void *re, void *im) {
 
 int zb_state_set_raw(zb_backend *h, const void *re, const void *im) {
     return guarded([&] {
+        h->win_ops.clear();  // the state is being overwritten
+        h->win_data.clear();
         check(hipMemcpyAsync(h->re, re, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy re");
         check(hipMemcpyAsync(h->im, im, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy im");
         h->sync();
@@ -487,6 +751,11 @@ int zb_state_set_raw(zb_backend *h, const void *re, const void *im) {
 }
 
 int zb_state_device_ptrs(zb_backend *h, void **re, void **im) {
+    const int rc = guarded([&] {
+        flush(h);
+        h->sync();
+    });
+    if (rc != ZB_OK) return rc;
     *re = h->re;
     *im = h->im;
     return ZB_OK;
@@ -494,6 +763,7 @@ int zb_state_device_ptrs(zb_backend *h, void **re, void **im) {
 
 int zb_statevector(zb_backend *h, double *out) {
     return guarded([&] {
+        flush(h);
         const uint64_t n = h->len;
         if (h->prec == 64) {
             std::vector<double> re(n), im(n);
@@ -518,7 +788,51 @@ double zb_norm_sqr(zb_backend *h) {
 }
 
 int zb_finish(zb_backend *h) {
-    return guarded([&] { h->sync(); });
+    return guarded([&] {
+        flush(h);
+        h->sync();
+    });
+}
+
+int zb_tune(zb_backend *h, const char *key, int64_t value) {
+    const std::string k(key ? key : "");
+    return guarded([&] {
+        flush(h);
+        if (k == "prefetch") h->prefetch = value != 0;
+        else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
+        else throw std::runtime_error("zb_tune: unknown key or bad value: " + k);
+    });
+}
+
+int zb_profile_begin(zb_backend *h) {
+    return guarded([&] {
+        flush(h);
+        h->sync();
+        h->recs.clear();
+        h->ev_next = 0;
+        h->prof = true;
+    });
+}
+
+int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms, double *bytes, int64_t cap) {
+    int64_t n = 0;
+    const int rc = guarded([&] {
+        flush(h);
+        h->sync();
+        h->prof = false;
+        for (const auto &r : h->recs) {
+            if (n >= cap) break;
+            float t = 0.f;
+            check(hipEventElapsedTime(&t, r.a, r.b), "hipEventElapsedTime");
+            kernel[n] = r.kernel;
+            ms[n] = (double)t;
+            bytes[n] = r.bytes;
+            ++n;
+        }
+        h->recs.clear();
+        h->ev_next = 0;
+    });
+    return rc == ZB_OK ? n : rc;
 }
 
 int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out) {
@@ -527,6 +841,8 @@ int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out) {
         return ZB_ERR_ARG;
     }
     return guarded([&] {
+        flush(a);
+        flush(b);
         b->sync();
         unsigned long long *d = reinterpret_cast<unsigned long long *>(a->scratch_get(8));
         check(hipMemsetAsync(d, 0, 8, a->stream), "hipMemsetAsync");

@@ -235,6 +235,360 @@ __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__restrict_
     if (threadIdx.x == 0) out[blockIdx.x] = tot;
 }
 
+// ------------------------------------------------------------ tiled window
+// One HBM pass applies a window of consecutive ops.  A tile is the 2^m
+// amplitudes sharing every index bit outside the window's qubit set Q
+// (|Q| = m <= 12, Q sorted; local bit i <-> qubit Q[i]; Q holds the lowest
+// qubits, so a tile's HBM accesses are 128-byte runs).  The tile is staged in
+// LDS (XOR-swizzled against bank conflicts).  The window is cut into
+// stages: a stage fixes 4 local "slot" bits; each of the 256 lanes gathers
+// the 16 amplitudes that differ only in those bits into registers, applies
+// the stage's ops there with exactly the per-op kernels' arithmetic, and
+// scatters them back.  The next tile's HBM loads are issued before the
+// current tile's stages run.  Ops admitted to a window: 1q dense and
+// dense k <= 4 whose qubits are slots of one stage; diagonals on any qubits
+// (elementwise: bits outside Q are fixed per tile); CX whose target is a
+// slot (its control may be a slot, another local bit, or fixed per tile).
+constexpr int kTileBits = 12;
+constexpr int kTile = 1 << kTileBits;
+constexpr int kSlots = 4;
+constexpr int kRegs = 1 << kSlots;
+constexpr uint32_t kFixedBit = 0x80000000u;  // table bit outside Q: qubit in the low bits
+
+enum WinKind : uint32_t { kW1q = 0, kWDense = 1, kWDiag = 2, kWCx = 3 };
+enum CxCtrl : uint32_t { kCtrlSlot = 0, kCtrlLocal = 1, kCtrlFixed = 2 };
+
+struct WinOp {
+    uint32_t kind;
+    uint32_t sel;       // 1q: slot; dense: slot mask; cx: target slot | ctrl slot << 2 | mode << 4
+    uint32_t aux;       // cx: control local bit (mode 1) or qubit (mode 2)
+    uint32_t k;         // diag: table bits
+    uint32_t data_off;  // offset into the window's data block (in T)
+    uint32_t pad[3];
+    uint32_t bits[kMaxDiagK];  // diag: local bit of table bit b, or kFixedBit | qubit
+};
+
+struct WinStage {
+    uint32_t slot[kSlots];  // ascending local bit positions
+    uint32_t op_begin, op_end;
+};
+
+struct WinHeader {
+    uint32_t m, nstages;
+    uint32_t q[kTileBits];
+};
+
+__device__ __forceinline__ int swz(int l) { return l ^ ((l >> 4) & 15) ^ ((l >> 8) & 15); }
+
+template <typename T, int S>
+__device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
+    const T m00r = d[0], m01r = d[1], m10r = d[2], m11r = d[3];
+    const T m00i = d[4], m01i = d[5], m10i = d[6], m11i = d[7];
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) {
+        if ((r >> S) & 1) continue;
+        const int a = r, c = r | (1 << S);
+        const T ar = xr[a], ai = xi[a], br = xr[c], bi = xi[c];
+        xr[a] = m00r * ar - m00i * ai + m01r * br - m01i * bi;
+        xi[a] = m00r * ai + m00i * ar + m01r * bi + m01i * br;
+        xr[c] = m10r * ar - m10i * ai + m11r * br - m11i * bi;
+        xi[c] = m10r * ai + m10i * ar + m11r * bi + m11i * br;
+    }
+}
+
+// Dense op on the slots of mask M (matrix local bit b <-> b-th set bit).
+template <typename T, int M>
+__device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d) {
+    constexpr int K = __builtin_popcount(M);
+    constexpr int D = 1 << K;
+    int pos[K > 0 ? K : 1];
+    {
+        int b = 0;
+#pragma unroll
+        for (int s = 0; s < kSlots; ++s)
+            if ((M >> s) & 1) pos[b++] = s;
+    }
+#pragma unroll
+    for (int g = 0; g < kRegs; ++g) {
+        if (g & M) continue;  // g enumerates the register bits outside M
+        T yr[D], yi[D], vr[D], vi[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            int r = g;
+#pragma unroll
+            for (int b = 0; b < K; ++b)
+                if ((j >> b) & 1) r |= 1 << pos[b];
+            vr[j] = xr[r];
+            vi[j] = xi[r];
+        }
+#pragma unroll
+        for (int row = 0; row < D; ++row) {
+            T ar = T(0), ai = T(0);
+#pragma unroll
+            for (int c = 0; c < D; ++c) {
+                const T wr = d[row * D + c], wi = d[D * D + row * D + c];
+                ar = ar + wr * vr[c] - wi * vi[c];
+                ai = ai + wr * vi[c] + wi * vr[c];
+            }
+            yr[row] = ar;
+            yi[row] = ai;
+        }
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            int r = g;
+#pragma unroll
+            for (int b = 0; b < K; ++b)
+                if ((j >> b) & 1) r |= 1 << pos[b];
+            xr[r] = yr[j];
+            xi[r] = yi[j];
+        }
+    }
+}
+
+template <typename T, int ST, int SC>
+__device__ __forceinline__ void reg_cx_slot(T *xr, T *xi) {
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) {
+        if (((r >> ST) & 1) || !((r >> SC) & 1)) continue;
+        const int c = r | (1 << ST);
+        const T t0 = xr[r], t1 = xi[r];
+        xr[r] = xr[c];
+        xi[r] = xi[c];
+        xr[c] = t0;
+        xi[c] = t1;
+    }
+}
+
+template <typename T, int ST>
+__device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
+#pragma unroll
+    for (int r = 0; r < kRegs; ++r) {
+        if ((r >> ST) & 1) continue;
+        const int c = r | (1 << ST);
+        const T a0 = xr[r], a1 = xi[r], b0 = xr[c], b1 = xi[c];
+        xr[r] = on ? b0 : a0;
+        xi[r] = on ? b1 : a1;
+        xr[c] = on ? a0 : b0;
+        xi[c] = on ? a1 : b1;
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ void reg_dense_dispatch(T *xr, T *xi, uint32_t mask, const T *__restrict__ d) {
+    switch (mask) {
+        case 0x3: reg_dense<T, 0x3>(xr, xi, d); break;
+        case 0x5: reg_dense<T, 0x5>(xr, xi, d); break;
+        case 0x6: reg_dense<T, 0x6>(xr, xi, d); break;
+        case 0x9: reg_dense<T, 0x9>(xr, xi, d); break;
+        case 0xA: reg_dense<T, 0xA>(xr, xi, d); break;
+        case 0xC: reg_dense<T, 0xC>(xr, xi, d); break;
+        case 0x7: reg_dense<T, 0x7>(xr, xi, d); break;
+        case 0xB: reg_dense<T, 0xB>(xr, xi, d); break;
+        case 0xD: reg_dense<T, 0xD>(xr, xi, d); break;
+        case 0xE: reg_dense<T, 0xE>(xr, xi, d); break;
+        case 0xF: reg_dense<T, 0xF>(xr, xi, d); break;
+    }
+}
+
+// kDense: the window holds dense k >= 2 ops; kPrefetch: the next tile's HBM
+// loads go to registers while the current tile computes (separate instances
+// keep the register budget of the common 1q/CX/diagonal windows low).
+template <typename T, bool kDense, bool kPrefetch>
+__global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__restrict__ im,
+                                                   const WinHeader *__restrict__ hdr,
+                                                   const WinStage *__restrict__ stages,
+                                                   const WinOp *__restrict__ ops, const T *__restrict__ data,
+                                                   uint64_t tiles) {
+    __shared__ T sre[kTile], sim[kTile];
+    __shared__ uint64_t shi[kRegs];
+    __shared__ uint32_t sq[kTileBits];
+    const int m = (int)hdr->m, L = 1 << m, nst = (int)hdr->nstages;
+    const int t = threadIdx.x;
+    const bool lane_on = t < (L >> kSlots);  // a lane owns 16 amplitudes per stage
+    if (t < kTileBits) sq[t] = t < m ? hdr->q[t] : 0;
+    if (t < kRegs) {
+        uint64_t s = 0;
+        for (int i = 8; i < m; ++i)
+            if ((t >> (i - 8)) & 1) s |= 1ull << hdr->q[i];
+        shi[t] = s;
+    }
+    // HBM <-> LDS: lane t moves local indices l = t + 256*j, whose global
+    // offsets are slo (the lane's low 8 local bits) | shi[j] (the high ones).
+    uint64_t slo = 0;
+    for (int i = 0; i < 8 && i < m; ++i)
+        if ((t >> i) & 1) slo |= 1ull << hdr->q[i];
+    __syncthreads();
+    auto tile_base = [&](uint64_t tile) {
+        uint64_t b = tile;
+        for (int i = 0; i < m; ++i) b = insert_zero(b, sq[i]);
+        return b;
+    };
+    T pr[kPrefetch ? kRegs : 1], pi[kPrefetch ? kRegs : 1];
+    uint64_t tile = blockIdx.x;
+    uint64_t base = tile < tiles ? tile_base(tile) : 0;
+    if (kPrefetch && tile < tiles) {
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) {
+            const int l = t + kBlock * j;
+            if (l < L) {
+                const uint64_t g = base | slo | shi[j];
+                pr[kPrefetch ? j : 0] = re[g];
+                pi[kPrefetch ? j : 0] = im[g];
+            }
+        }
+    }
+    while (tile < tiles) {
+        if constexpr (kPrefetch) {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) {
+                const int l = t + kBlock * j;
+                if (l < L) sre[swz(l)] = pr[j], sim[swz(l)] = pi[j];
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < kRegs; ++j) {
+                const int l = t + kBlock * j;
+                if (l < L) {
+                    const uint64_t g = base | slo | shi[j];
+                    sre[swz(l)] = re[g];
+                    sim[swz(l)] = im[g];
+                }
+            }
+        }
+        __syncthreads();
+        const uint64_t next = tile + gridDim.x;
+        const uint64_t nbase = next < tiles ? tile_base(next) : 0;
+        if constexpr (kPrefetch) {
+            if (next < tiles) {
+#pragma unroll
+                for (int j = 0; j < kRegs; ++j) {
+                    const int l = t + kBlock * j;
+                    if (l < L) {
+                        const uint64_t g = nbase | slo | shi[j];
+                        pr[j] = re[g];
+                        pi[j] = im[g];
+                    }
+                }
+            }
+        }
+        for (int s = 0; s < nst; ++s) {
+            const WinStage st = stages[s];
+            if (lane_on) {
+                int lb = t;
+#pragma unroll
+                for (int i = 0; i < kSlots; ++i) lb = (int)insert_zero((uint64_t)lb, st.slot[i]);
+                T xr[kRegs], xi[kRegs];
+#pragma unroll
+                for (int r = 0; r < kRegs; ++r) {
+                    int o = lb;
+#pragma unroll
+                    for (int i = 0; i < kSlots; ++i)
+                        if ((r >> i) & 1) o |= 1 << st.slot[i];
+                    xr[r] = sre[swz(o)];
+                    xi[r] = sim[swz(o)];
+                }
+                for (uint32_t o = st.op_begin; o < st.op_end; ++o) {
+                    const WinOp op = ops[o];
+                    const T *d = data + op.data_off;
+                    switch (op.kind) {
+                        case kW1q:
+                            switch (op.sel) {
+                                case 0: reg_1q<T, 0>(xr, xi, d); break;
+                                case 1: reg_1q<T, 1>(xr, xi, d); break;
+                                case 2: reg_1q<T, 2>(xr, xi, d); break;
+                                case 3: reg_1q<T, 3>(xr, xi, d); break;
+                            }
+                            break;
+                        case kWDense:
+                            if constexpr (kDense) reg_dense_dispatch<T>(xr, xi, op.sel, d);
+                            break;
+                        case kWDiag: {
+                            // table index: bits fixed by the tile or by the lane, plus
+                            // the bits that vary across the lane's 16 registers
+                            const int k = (int)op.k, D = 1 << k;
+                            uint32_t jt = 0, jslot[kSlots] = {0, 0, 0, 0};
+                            for (int b = 0; b < k; ++b) {
+                                const uint32_t bb = op.bits[b];
+                                if (bb & kFixedBit) {
+                                    jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
+                                    continue;
+                                }
+                                bool is_slot = false;
+#pragma unroll
+                                for (int i = 0; i < kSlots; ++i)
+                                    if (st.slot[i] == bb) jslot[i] = 1u << b, is_slot = true;
+                                if (!is_slot) jt |= (uint32_t)((lb >> bb) & 1) << b;
+                            }
+#pragma unroll
+                            for (int r = 0; r < kRegs; ++r) {
+                                uint32_t j = jt;
+#pragma unroll
+                                for (int i = 0; i < kSlots; ++i)
+                                    if ((r >> i) & 1) j |= jslot[i];
+                                const T wr = d[j], wi = d[D + j];
+                                const T ar = xr[r], ai = xi[r];
+                                xr[r] = wr * ar - wi * ai;
+                                xi[r] = wr * ai + wi * ar;
+                            }
+                            break;
+                        }
+                        case kWCx: {
+                            const uint32_t st_ = op.sel & 3, sc = (op.sel >> 2) & 3, mode = op.sel >> 4;
+                            if (mode == kCtrlSlot) {
+                                switch (st_ * 4 + sc) {
+                                    case 1: reg_cx_slot<T, 0, 1>(xr, xi); break;
+                                    case 2: reg_cx_slot<T, 0, 2>(xr, xi); break;
+                                    case 3: reg_cx_slot<T, 0, 3>(xr, xi); break;
+                                    case 4: reg_cx_slot<T, 1, 0>(xr, xi); break;
+                                    case 6: reg_cx_slot<T, 1, 2>(xr, xi); break;
+                                    case 7: reg_cx_slot<T, 1, 3>(xr, xi); break;
+                                    case 8: reg_cx_slot<T, 2, 0>(xr, xi); break;
+                                    case 9: reg_cx_slot<T, 2, 1>(xr, xi); break;
+                                    case 11: reg_cx_slot<T, 2, 3>(xr, xi); break;
+                                    case 12: reg_cx_slot<T, 3, 0>(xr, xi); break;
+                                    case 13: reg_cx_slot<T, 3, 1>(xr, xi); break;
+                                    case 14: reg_cx_slot<T, 3, 2>(xr, xi); break;
+                                }
+                            } else {
+                                const bool on = mode == kCtrlLocal ? ((lb >> op.aux) & 1) : ((base >> op.aux) & 1);
+                                switch (st_) {
+                                    case 0: reg_cx_all<T, 0>(xr, xi, on); break;
+                                    case 1: reg_cx_all<T, 1>(xr, xi, on); break;
+                                    case 2: reg_cx_all<T, 2>(xr, xi, on); break;
+                                    case 3: reg_cx_all<T, 3>(xr, xi, on); break;
+                                }
+                            }
+                            break;
+                        }
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < kRegs; ++r) {
+                    int o = lb;
+#pragma unroll
+                    for (int i = 0; i < kSlots; ++i)
+                        if ((r >> i) & 1) o |= 1 << st.slot[i];
+                    sre[swz(o)] = xr[r];
+                    sim[swz(o)] = xi[r];
+                }
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int j = 0; j < kRegs; ++j) {
+            const int l = t + kBlock * j;
+            if (l < L) {
+                const uint64_t g = base | slo | shi[j];
+                re[g] = sre[swz(l)];
+                im[g] = sim[swz(l)];
+            }
+        }
+        __syncthreads();
+        tile = next;
+        base = nbase;
+    }
+}
+
 // max_i |a_i - b_i| (complex modulus, f64) between two states of possibly
 // different precision; non-negative doubles order like their bit patterns,
 // so the grid combines block maxima with an integer atomicMax.
