@@ -69,7 +69,8 @@ struct zb_backend {
     };
     uint64_t win_mask = 0;
     std::vector<POp> win_ops;
-    std::vector<unsigned char> win_data;
+    std::vector<unsigned char> win_data;    // 1q / dense matrices (copied into LDS)
+    std::vector<unsigned char> win_tables;  // diagonal tables (read from global)
     // launch profiling: HIP events around every state-sweep launch
     struct Rec {
         int kernel;
@@ -253,30 +254,43 @@ void exact_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
 // set Q (the lowest `low` qubits plus every qubit an op needs inside the
 // tile) stays within the tile width; otherwise the window is flushed as one
 // k_window pass.  A window holding a single op runs the per-op kernel, and
-// so do dense ops on 5 or 6 qubits and every op of a state under 8 qubits.
+// so do dense ops on 5 or 6 qubits and every op of a state under 12 qubits.
 uint32_t tile_bits(const zb_backend *h) { return std::min<uint32_t>(h->n, dev::kTileBits); }
-bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= 8; }
+// Tiled windows need n >= 12 (full tiles) and 32-bit lane offsets (n <= 33).
+bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= (uint32_t)dev::kTileBits && h->n <= 33; }
 
 uint64_t low_mask(const zb_backend *h) {
     const uint32_t low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, h->n);
     return (1ull << low) - 1;
 }
 
+// Bytes of the descriptor blob for a window of `nops` ops and `data`
+// bytes of matrices (stages <= ops).
+size_t desc_bytes(size_t nops, size_t data) {
+    auto pad = [](size_t x) { return (x + 15) & ~size_t(15); };
+    return pad(sizeof(dev::WinHeader)) + pad(nops * sizeof(dev::WinStage)) + pad(nops * sizeof(dev::WinOp)) + data;
+}
+
 template <typename T>
 void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint64_t need, const double *src,
              size_t n_complex) {
     const uint64_t m = tile_bits(h);
-    if (!h->win_ops.empty() && (uint64_t)__builtin_popcountll(h->win_mask | need) > m) flush(h);
+    const bool table = kind == dev::kWDiag;
+    const size_t add = table ? 0 : 2 * n_complex * sizeof(T);
+    if (!h->win_ops.empty() && ((uint64_t)__builtin_popcountll(h->win_mask | need) > m ||
+                                desc_bytes(h->win_ops.size() + 1, h->win_data.size() + add) > dev::kDescBytes))
+        flush(h);
     if (h->win_ops.empty()) h->win_mask = low_mask(h);
     h->win_mask |= need;
     zb_backend::POp op{};
     op.kind = kind;
     op.k = k;
     for (uint32_t b = 0; b < k; ++b) op.q[b] = qs[b];
-    op.data_off = (uint32_t)(h->win_data.size() / sizeof(T));
-    const size_t at = h->win_data.size();
-    h->win_data.resize(at + 2 * n_complex * sizeof(T));
-    T *dst = reinterpret_cast<T *>(h->win_data.data() + at);
+    std::vector<unsigned char> &buf = table ? h->win_tables : h->win_data;
+    op.data_off = (uint32_t)(buf.size() / sizeof(T));
+    const size_t at = buf.size();
+    buf.resize(at + 2 * n_complex * sizeof(T));
+    T *dst = reinterpret_cast<T *>(buf.data() + at);
     for (size_t i = 0; i < n_complex; ++i) {
         dst[i] = (T)src[2 * i];  // T::fr
         dst[n_complex + i] = (T)src[2 * i + 1];
@@ -288,12 +302,14 @@ template <typename T>
 void flush_T(zb_backend *h) {
     if (h->win_ops.size() == 1) {  // one op: the per-op kernel is the cheaper pass
         const zb_backend::POp op = h->win_ops[0];
-        const T *d = reinterpret_cast<const T *>(h->win_data.data()) + op.data_off;
+        const T *d = reinterpret_cast<const T *>(op.kind == dev::kWDiag ? h->win_tables.data() : h->win_data.data()) +
+                     op.data_off;
         const size_t len = op.kind == dev::kWDiag ? (1u << op.k) : (1u << (2 * op.k));
         std::vector<double> inter(2 * len);
         for (size_t i = 0; i < len; ++i) inter[2 * i] = (double)d[i], inter[2 * i + 1] = (double)d[len + i];
         h->win_ops.clear();
         h->win_data.clear();
+        h->win_tables.clear();
         if (op.kind == dev::kWCx) exact_cx_T<T>(h, op.q[1], op.q[0]);
         else if (op.kind == dev::kWDiag) exact_diag_T<T>(h, op.q, op.k, inter.data());
         else exact_unitary_T<T>(h, op.q, op.k, inter.data());
@@ -385,35 +401,42 @@ void flush_T(zb_backend *h) {
             }
         }
     }
-    auto pad = [](size_t x) { return (x + 255) & ~size_t(255); };
+    auto pad = [](size_t x) { return (x + 15) & ~size_t(15); };
     const size_t hb = pad(sizeof(hdr)), sb = pad(stages.size() * sizeof(dev::WinStage)),
                  ob = pad(wops.size() * sizeof(dev::WinOp));
-    std::vector<unsigned char> blob(hb + sb + ob + h->win_data.size());
+    const size_t total = hb + sb + ob + pad(h->win_data.size());
+    hdr.nops = (uint32_t)wops.size();
+    hdr.desc_words = (uint32_t)(total / 4);
+    hdr.stage_off = (uint32_t)(hb / 4);
+    hdr.op_off = (uint32_t)((hb + sb) / 4);
+    hdr.data_off = (uint32_t)((hb + sb + ob) / 4);
+    if (total > (size_t)dev::kDescBytes) throw std::runtime_error("window descriptor exceeds its size limit");
+    std::vector<unsigned char> blob(total, 0);
     std::memcpy(blob.data(), &hdr, sizeof(hdr));
     std::memcpy(blob.data() + hb, stages.data(), stages.size() * sizeof(dev::WinStage));
     std::memcpy(blob.data() + hb + sb, wops.data(), wops.size() * sizeof(dev::WinOp));
     std::memcpy(blob.data() + hb + sb + ob, h->win_data.data(), h->win_data.size());
-    unsigned char *d = static_cast<unsigned char *>(h->upload(blob.data(), blob.size()));
+    const uint32_t *dblob = static_cast<const uint32_t *>(h->upload(blob.data(), blob.size()));
+    const T *dtab = h->win_tables.empty()
+                        ? nullptr
+                        : static_cast<const T *>(h->upload(h->win_tables.data(), h->win_tables.size()));
     const uint64_t tiles = h->len >> m;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, h->win_grid);
-    const auto *dh = reinterpret_cast<const dev::WinHeader *>(d);
-    const auto *ds = reinterpret_cast<const dev::WinStage *>(d + hb);
-    const auto *dop = reinterpret_cast<const dev::WinOp *>(d + hb + sb);
-    const auto *dd = reinterpret_cast<const T *>(d + hb + sb + ob);
     h->timed(ZB_K_WINDOW, h->sweep_bytes(), [&] {
         const dim3 g(grid), b(dev::kBlock);
         if (dense && h->prefetch)
-            hipLaunchKernelGGL((dev::k_window<T, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+            hipLaunchKernelGGL((dev::k_window<T, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
         else if (dense)
-            hipLaunchKernelGGL((dev::k_window<T, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+            hipLaunchKernelGGL((dev::k_window<T, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
         else if (h->prefetch)
-            hipLaunchKernelGGL((dev::k_window<T, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+            hipLaunchKernelGGL((dev::k_window<T, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
         else
-            hipLaunchKernelGGL((dev::k_window<T, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dh, ds, dop, dd, tiles);
+            hipLaunchKernelGGL((dev::k_window<T, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
     });
     check(hipGetLastError(), "k_window");
     h->win_ops.clear();
     h->win_data.clear();
+    h->win_tables.clear();
 }
 
 void flush(zb_backend *h) {
@@ -705,6 +728,7 @@ int zb_reset_all(zb_backend *h) {
     return guarded([&] {
         h->win_ops.clear();  // pending ops act on a state that is being overwritten
         h->win_data.clear();
+        h->win_tables.clear();
         if (h->prec == 64) launch_zero<double>(h);
         else launch_zero<float>(h);
         h->sync();
@@ -744,6 +768,7 @@ int zb_state_set_raw(zb_backend *h, const void *re, const void *im) {
     return guarded([&] {
         h->win_ops.clear();  // the state is being overwritten
         h->win_data.clear();
+        h->win_tables.clear();
         check(hipMemcpyAsync(h->re, re, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy re");
         check(hipMemcpyAsync(h->im, im, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy im");
         h->sync();

@@ -236,24 +236,26 @@ __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__restrict_
 }
 
 // ------------------------------------------------------------ tiled window
-// One HBM pass applies a window of consecutive ops.  A tile is the 2^m
+// One HBM pass applies a window of consecutive ops.  A tile is the 2^12
 // amplitudes sharing every index bit outside the window's qubit set Q
-// (|Q| = m <= 12, Q sorted; local bit i <-> qubit Q[i]; Q holds the lowest
-// qubits, so a tile's HBM accesses are 128-byte runs).  The tile is staged in
-// LDS (XOR-swizzled against bank conflicts).  The window is cut into
+// (|Q| = 12, Q sorted; local bit i <-> qubit Q[i]; Q holds the lowest
+// qubits, so a tile's HBM accesses are 128-byte runs).  The tile is staged
+// in LDS with one pad element per 16 (bank spread).  The window is cut into
 // stages: a stage fixes 4 local "slot" bits; each of the 256 lanes gathers
 // the 16 amplitudes that differ only in those bits into registers, applies
 // the stage's ops there with exactly the per-op kernels' arithmetic, and
-// scatters them back.  The next tile's HBM loads are issued before the
-// current tile's stages run.  Ops admitted to a window: 1q dense and
-// dense k <= 4 whose qubits are slots of one stage; diagonals on any qubits
+// scatters them back.  Ops admitted to a window: 1q dense and dense k <= 4
+// whose qubits are slots of one stage; diagonals on any qubits
 // (elementwise: bits outside Q are fixed per tile); CX whose target is a
 // slot (its control may be a slot, another local bit, or fixed per tile).
+// Window descriptors are uniform and read with scalar loads.
 constexpr int kTileBits = 12;
 constexpr int kTile = 1 << kTileBits;
+constexpr int kTilePad = kTile + (kTile >> 4);
 constexpr int kSlots = 4;
 constexpr int kRegs = 1 << kSlots;
 constexpr uint32_t kFixedBit = 0x80000000u;  // table bit outside Q: qubit in the low bits
+constexpr int kDescBytes = 65536;           // window descriptor blob limit
 
 enum WinKind : uint32_t { kW1q = 0, kWDense = 1, kWDiag = 2, kWCx = 3 };
 enum CxCtrl : uint32_t { kCtrlSlot = 0, kCtrlLocal = 1, kCtrlFixed = 2 };
@@ -263,7 +265,7 @@ struct WinOp {
     uint32_t sel;       // 1q: slot; dense: slot mask; cx: target slot | ctrl slot << 2 | mode << 4
     uint32_t aux;       // cx: control local bit (mode 1) or qubit (mode 2)
     uint32_t k;         // diag: table bits
-    uint32_t data_off;  // offset into the window's data block (in T)
+    uint32_t data_off;  // offset of the matrix (blob, in T) or diagonal table (tables, in T)
     uint32_t pad[3];
     uint32_t bits[kMaxDiagK];  // diag: local bit of table bit b, or kFixedBit | qubit
 };
@@ -274,11 +276,12 @@ struct WinStage {
 };
 
 struct WinHeader {
-    uint32_t m, nstages;
+    uint32_t m, nstages, nops, desc_words;
+    uint32_t stage_off, op_off, data_off, pad;  // word offsets inside the blob
     uint32_t q[kTileBits];
 };
 
-__device__ __forceinline__ int swz(int l) { return l ^ ((l >> 4) & 15) ^ ((l >> 8) & 15); }
+__device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
 
 template <typename T, int S>
 __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
@@ -393,67 +396,64 @@ __device__ __forceinline__ void reg_dense_dispatch(T *xr, T *xi, uint32_t mask, 
 // kDense: the window holds dense k >= 2 ops; kPrefetch: the next tile's HBM
 // loads go to registers while the current tile computes (separate instances
 // keep the register budget of the common 1q/CX/diagonal windows low).
+// Global addressing: row j of a tile starts at the uniform element offset
+// base | hi(j); a lane adds its 32-bit offset lo (its low 8 local bits),
+// which needs every Q[0..7] < 31 - log2(sizeof(T)) (the host checks).
 template <typename T, bool kDense, bool kPrefetch>
 __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__restrict__ im,
-                                                   const WinHeader *__restrict__ hdr,
-                                                   const WinStage *__restrict__ stages,
-                                                   const WinOp *__restrict__ ops, const T *__restrict__ data,
+                                                   const uint32_t *__restrict__ blob, const T *__restrict__ tables,
                                                    uint64_t tiles) {
-    __shared__ T sre[kTile], sim[kTile];
-    __shared__ uint64_t shi[kRegs];
-    __shared__ uint32_t sq[kTileBits];
-    const int m = (int)hdr->m, L = 1 << m, nst = (int)hdr->nstages;
+    __shared__ T sre[kTilePad], sim[kTilePad];
     const int t = threadIdx.x;
-    const bool lane_on = t < (L >> kSlots);  // a lane owns 16 amplitudes per stage
-    if (t < kTileBits) sq[t] = t < m ? hdr->q[t] : 0;
-    if (t < kRegs) {
-        uint64_t s = 0;
-        for (int i = 8; i < m; ++i)
-            if ((t >> (i - 8)) & 1) s |= 1ull << hdr->q[i];
-        shi[t] = s;
-    }
-    // HBM <-> LDS: lane t moves local indices l = t + 256*j, whose global
-    // offsets are slo (the lane's low 8 local bits) | shi[j] (the high ones).
-    uint64_t slo = 0;
-    for (int i = 0; i < 8 && i < m; ++i)
-        if ((t >> i) & 1) slo |= 1ull << hdr->q[i];
-    __syncthreads();
+    const WinHeader *hdr = reinterpret_cast<const WinHeader *>(blob);
+    const WinStage *stages = reinterpret_cast<const WinStage *>(blob + hdr->stage_off);
+    const WinOp *ops = reinterpret_cast<const WinOp *>(blob + hdr->op_off);
+    const T *mats = reinterpret_cast<const T *>(blob + hdr->data_off);
+    const int nst = (int)hdr->nstages;
+    uint32_t lo = 0;  // the lane's element offset within a row
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+        if ((t >> i) & 1) lo |= 1u << hdr->q[i];
+    const uint32_t lob = lo * (uint32_t)sizeof(T);  // < 2^32 (see above)
+    auto at = [&](T *p, uint64_t h) -> T & { return *reinterpret_cast<T *>(reinterpret_cast<char *>(p + h) + lob); };
+    const int lt = padded(t);  // LDS slot of local index t (+ 272*j for row j)
     auto tile_base = [&](uint64_t tile) {
         uint64_t b = tile;
-        for (int i = 0; i < m; ++i) b = insert_zero(b, sq[i]);
+#pragma unroll
+        for (int i = 0; i < kTileBits; ++i) b = insert_zero(b, hdr->q[i]);
         return b;
     };
-    T pr[kPrefetch ? kRegs : 1], pi[kPrefetch ? kRegs : 1];
+    auto row = [&](uint64_t base, int j) {
+        uint64_t h = base;
+#pragma unroll
+        for (int i = 8; i < kTileBits; ++i)
+            if ((j >> (i - 8)) & 1) h |= 1ull << hdr->q[i];
+        return h;
+    };
+    T pr[kRegs], pi[kRegs];
     uint64_t tile = blockIdx.x;
     uint64_t base = tile < tiles ? tile_base(tile) : 0;
     if (kPrefetch && tile < tiles) {
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) {
-            const int l = t + kBlock * j;
-            if (l < L) {
-                const uint64_t g = base | slo | shi[j];
-                pr[kPrefetch ? j : 0] = re[g];
-                pi[kPrefetch ? j : 0] = im[g];
-            }
+            const uint64_t h = row(base, j);
+            pr[j] = at(re, h);
+            pi[j] = at(im, h);
         }
     }
     while (tile < tiles) {
-        if constexpr (kPrefetch) {
+        if constexpr (!kPrefetch) {
 #pragma unroll
             for (int j = 0; j < kRegs; ++j) {
-                const int l = t + kBlock * j;
-                if (l < L) sre[swz(l)] = pr[j], sim[swz(l)] = pi[j];
+                const uint64_t h = row(base, j);
+                pr[j] = at(re, h);
+                pi[j] = at(im, h);
             }
-        } else {
+        }
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) {
-                const int l = t + kBlock * j;
-                if (l < L) {
-                    const uint64_t g = base | slo | shi[j];
-                    sre[swz(l)] = re[g];
-                    sim[swz(l)] = im[g];
-                }
-            }
+        for (int j = 0; j < kRegs; ++j) {
+            sre[lt + (kBlock + kBlock / 16) * j] = pr[j];
+            sim[lt + (kBlock + kBlock / 16) * j] = pi[j];
         }
         __syncthreads();
         const uint64_t next = tile + gridDim.x;
@@ -462,126 +462,115 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
             if (next < tiles) {
 #pragma unroll
                 for (int j = 0; j < kRegs; ++j) {
-                    const int l = t + kBlock * j;
-                    if (l < L) {
-                        const uint64_t g = nbase | slo | shi[j];
-                        pr[j] = re[g];
-                        pi[j] = im[g];
-                    }
+                    const uint64_t h = row(nbase, j);
+                    pr[j] = at(re, h);
+                    pi[j] = at(im, h);
                 }
             }
         }
         for (int s = 0; s < nst; ++s) {
-            const WinStage st = stages[s];
-            if (lane_on) {
-                int lb = t;
+            const WinStage *st = stages + s;
+            const int s0 = (int)st->slot[0], s1 = (int)st->slot[1], s2 = (int)st->slot[2], s3 = (int)st->slot[3];
+            const int lb = (int)insert_zero(insert_zero(insert_zero(insert_zero((uint64_t)t, s0), s1), s2), s3);
+            const int plb = padded(lb);
+            T xr[kRegs], xi[kRegs];
+            // lb and the register bits are disjoint, so padded(lb | c) =
+            // padded(lb) + padded(c): a lane base plus a uniform offset
 #pragma unroll
-                for (int i = 0; i < kSlots; ++i) lb = (int)insert_zero((uint64_t)lb, st.slot[i]);
-                T xr[kRegs], xi[kRegs];
-#pragma unroll
-                for (int r = 0; r < kRegs; ++r) {
-                    int o = lb;
-#pragma unroll
-                    for (int i = 0; i < kSlots; ++i)
-                        if ((r >> i) & 1) o |= 1 << st.slot[i];
-                    xr[r] = sre[swz(o)];
-                    xi[r] = sim[swz(o)];
-                }
-                for (uint32_t o = st.op_begin; o < st.op_end; ++o) {
-                    const WinOp op = ops[o];
-                    const T *d = data + op.data_off;
-                    switch (op.kind) {
-                        case kW1q:
-                            switch (op.sel) {
-                                case 0: reg_1q<T, 0>(xr, xi, d); break;
-                                case 1: reg_1q<T, 1>(xr, xi, d); break;
-                                case 2: reg_1q<T, 2>(xr, xi, d); break;
-                                case 3: reg_1q<T, 3>(xr, xi, d); break;
-                            }
-                            break;
-                        case kWDense:
-                            if constexpr (kDense) reg_dense_dispatch<T>(xr, xi, op.sel, d);
-                            break;
-                        case kWDiag: {
-                            // table index: bits fixed by the tile or by the lane, plus
-                            // the bits that vary across the lane's 16 registers
-                            const int k = (int)op.k, D = 1 << k;
-                            uint32_t jt = 0, jslot[kSlots] = {0, 0, 0, 0};
-                            for (int b = 0; b < k; ++b) {
-                                const uint32_t bb = op.bits[b];
-                                if (bb & kFixedBit) {
-                                    jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
-                                    continue;
-                                }
-                                bool is_slot = false;
-#pragma unroll
-                                for (int i = 0; i < kSlots; ++i)
-                                    if (st.slot[i] == bb) jslot[i] = 1u << b, is_slot = true;
-                                if (!is_slot) jt |= (uint32_t)((lb >> bb) & 1) << b;
-                            }
-#pragma unroll
-                            for (int r = 0; r < kRegs; ++r) {
-                                uint32_t j = jt;
-#pragma unroll
-                                for (int i = 0; i < kSlots; ++i)
-                                    if ((r >> i) & 1) j |= jslot[i];
-                                const T wr = d[j], wi = d[D + j];
-                                const T ar = xr[r], ai = xi[r];
-                                xr[r] = wr * ar - wi * ai;
-                                xi[r] = wr * ai + wi * ar;
-                            }
-                            break;
+            for (int r = 0; r < kRegs; ++r) {
+                const int c = ((r & 1) << s0) | (((r >> 1) & 1) << s1) | (((r >> 2) & 1) << s2) | (((r >> 3) & 1) << s3);
+                xr[r] = sre[plb + padded(c)];
+                xi[r] = sim[plb + padded(c)];
+            }
+            const int oe = (int)st->op_end;
+            for (int o = (int)st->op_begin; o < oe; ++o) {
+                const WinOp *op = ops + o;
+                const uint32_t kind = op->kind, sel = op->sel;
+                switch (kind) {
+                    case kW1q: {
+                        const T *d = mats + op->data_off;
+                        switch (sel) {
+                            case 0: reg_1q<T, 0>(xr, xi, d); break;
+                            case 1: reg_1q<T, 1>(xr, xi, d); break;
+                            case 2: reg_1q<T, 2>(xr, xi, d); break;
+                            case 3: reg_1q<T, 3>(xr, xi, d); break;
                         }
-                        case kWCx: {
-                            const uint32_t st_ = op.sel & 3, sc = (op.sel >> 2) & 3, mode = op.sel >> 4;
-                            if (mode == kCtrlSlot) {
-                                switch (st_ * 4 + sc) {
-                                    case 1: reg_cx_slot<T, 0, 1>(xr, xi); break;
-                                    case 2: reg_cx_slot<T, 0, 2>(xr, xi); break;
-                                    case 3: reg_cx_slot<T, 0, 3>(xr, xi); break;
-                                    case 4: reg_cx_slot<T, 1, 0>(xr, xi); break;
-                                    case 6: reg_cx_slot<T, 1, 2>(xr, xi); break;
-                                    case 7: reg_cx_slot<T, 1, 3>(xr, xi); break;
-                                    case 8: reg_cx_slot<T, 2, 0>(xr, xi); break;
-                                    case 9: reg_cx_slot<T, 2, 1>(xr, xi); break;
-                                    case 11: reg_cx_slot<T, 2, 3>(xr, xi); break;
-                                    case 12: reg_cx_slot<T, 3, 0>(xr, xi); break;
-                                    case 13: reg_cx_slot<T, 3, 1>(xr, xi); break;
-                                    case 14: reg_cx_slot<T, 3, 2>(xr, xi); break;
-                                }
-                            } else {
-                                const bool on = mode == kCtrlLocal ? ((lb >> op.aux) & 1) : ((base >> op.aux) & 1);
-                                switch (st_) {
-                                    case 0: reg_cx_all<T, 0>(xr, xi, on); break;
-                                    case 1: reg_cx_all<T, 1>(xr, xi, on); break;
-                                    case 2: reg_cx_all<T, 2>(xr, xi, on); break;
-                                    case 3: reg_cx_all<T, 3>(xr, xi, on); break;
-                                }
-                            }
-                            break;
+                        break;
+                    }
+                    case kWDense:
+                        if constexpr (kDense) reg_dense_dispatch<T>(xr, xi, sel, mats + op->data_off);
+                        break;
+                    case kWDiag: {
+                        // table index: bits fixed by the tile or by the lane, plus the
+                        // bits that vary across the lane's 16 registers
+                        const T *d = tables + op->data_off;
+                        const int k = (int)op->k, D = 1 << k;
+                        uint32_t jt = 0, j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+                        for (int b = 0; b < k; ++b) {
+                            const uint32_t bb = op->bits[b];
+                            if (bb & kFixedBit) jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
+                            else if ((int)bb == s0) j0 = 1u << b;
+                            else if ((int)bb == s1) j1 = 1u << b;
+                            else if ((int)bb == s2) j2 = 1u << b;
+                            else if ((int)bb == s3) j3 = 1u << b;
+                            else jt |= (uint32_t)((lb >> bb) & 1) << b;
                         }
+#pragma unroll
+                        for (int r = 0; r < kRegs; ++r) {
+                            const uint32_t j =
+                                jt | ((r & 1) ? j0 : 0) | ((r & 2) ? j1 : 0) | ((r & 4) ? j2 : 0) | ((r & 8) ? j3 : 0);
+                            const T wr = d[j], wi = d[D + j];
+                            const T ar = xr[r], ai = xi[r];
+                            xr[r] = wr * ar - wi * ai;
+                            xi[r] = wr * ai + wi * ar;
+                        }
+                        break;
+                    }
+                    case kWCx: {
+                        const uint32_t st_ = sel & 3, sc = (sel >> 2) & 3, mode = sel >> 4;
+                        if (mode == kCtrlSlot) {
+                            switch (st_ * 4 + sc) {
+                                case 1: reg_cx_slot<T, 0, 1>(xr, xi); break;
+                                case 2: reg_cx_slot<T, 0, 2>(xr, xi); break;
+                                case 3: reg_cx_slot<T, 0, 3>(xr, xi); break;
+                                case 4: reg_cx_slot<T, 1, 0>(xr, xi); break;
+                                case 6: reg_cx_slot<T, 1, 2>(xr, xi); break;
+                                case 7: reg_cx_slot<T, 1, 3>(xr, xi); break;
+                                case 8: reg_cx_slot<T, 2, 0>(xr, xi); break;
+                                case 9: reg_cx_slot<T, 2, 1>(xr, xi); break;
+                                case 11: reg_cx_slot<T, 2, 3>(xr, xi); break;
+                                case 12: reg_cx_slot<T, 3, 0>(xr, xi); break;
+                                case 13: reg_cx_slot<T, 3, 1>(xr, xi); break;
+                                case 14: reg_cx_slot<T, 3, 2>(xr, xi); break;
+                            }
+                        } else {
+                            const uint32_t aux = op->aux;
+                            const bool on = mode == kCtrlLocal ? ((lb >> aux) & 1) : ((base >> aux) & 1);
+                            switch (st_) {
+                                case 0: reg_cx_all<T, 0>(xr, xi, on); break;
+                                case 1: reg_cx_all<T, 1>(xr, xi, on); break;
+                                case 2: reg_cx_all<T, 2>(xr, xi, on); break;
+                                case 3: reg_cx_all<T, 3>(xr, xi, on); break;
+                            }
+                        }
+                        break;
                     }
                 }
+            }
 #pragma unroll
-                for (int r = 0; r < kRegs; ++r) {
-                    int o = lb;
-#pragma unroll
-                    for (int i = 0; i < kSlots; ++i)
-                        if ((r >> i) & 1) o |= 1 << st.slot[i];
-                    sre[swz(o)] = xr[r];
-                    sim[swz(o)] = xi[r];
-                }
+            for (int r = 0; r < kRegs; ++r) {
+                const int c = ((r & 1) << s0) | (((r >> 1) & 1) << s1) | (((r >> 2) & 1) << s2) | (((r >> 3) & 1) << s3);
+                sre[plb + padded(c)] = xr[r];
+                sim[plb + padded(c)] = xi[r];
             }
             __syncthreads();
         }
 #pragma unroll
         for (int j = 0; j < kRegs; ++j) {
-            const int l = t + kBlock * j;
-            if (l < L) {
-                const uint64_t g = base | slo | shi[j];
-                re[g] = sre[swz(l)];
-                im[g] = sim[swz(l)];
-            }
+            const T vr = sre[lt + (kBlock + kBlock / 16) * j], vi = sim[lt + (kBlock + kBlock / 16) * j];
+            const uint64_t h = row(base, j);
+            at(re, h) = vr;
+            at(im, h) = vi;
         }
         __syncthreads();
         tile = next;
