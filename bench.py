@@ -115,12 +115,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--n", type=int, default=30)
     ap.add_argument("--depth", type=int, default=40)
-    ap.add_argument("--fusion", type=int, default=5)
+    ap.add_argument("--fusion", type=int, default=1)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", type=int, default=1, help="0 exact per-op sweeps, 1 tiled")
     ap.add_argument("--cpu-ops", type=int, default=8)
     ap.add_argument("--also", default="", help="extra fusion:mode pairs to time once, e.g. 5:0,5:1")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain torch copy of 2^n f64 (re) + 2^n f64 (im), the known-size kernel "
+                    "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--tune", default="", help="tiled-kernel knobs, e.g. prefetch=1,win_grid=1024")
     args = ap.parse_args()
     for kv in [x for x in args.tune.split(",") if x]:
@@ -149,6 +152,16 @@ def main():
     t_compile = time.perf_counter() - t0
     ops = comp.ops()
     _log(f"compiled {comp.stats['input_gates']} gates -> {len(ops)} ops at fusion {args.fusion} in {t_compile:.2f} s")
+
+    if args.calibrate_copy:
+        src = torch.ones(2 << n, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(2):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        _log(f"calibration copy: {2 * src.numel() * 8} bytes moved per copy ({src.numel() * 8} read, same written)")
+        del src, dst
+        torch.cuda.empty_cache()
 
     be64, s64, cls64, bytes64 = run_gpu(n, comp, 64, args.mode, args.steps, args.warmup, torch)
     be32, s32, cls32, bytes32 = run_gpu(n, comp, 32, args.mode, args.steps, args.warmup, torch)
