@@ -48,7 +48,8 @@ struct zb_backend {
     void *re = nullptr, *im = nullptr;
     hipStream_t stream = nullptr;
     int mode = 1;
-    bool prefetch = false;  // tiled kernel variant: register prefetch of the next tile
+    bool prefetch = true;  // tiled kernel variant: register prefetch of the next tile
+    int slots = 0;          // tiled stage width: 3 (512-lane workgroups) or 4 (256 lanes); 0 = by precision
     unsigned win_grid = 2048;
     std::string name;
     // parameter arena
@@ -271,6 +272,21 @@ size_t desc_bytes(size_t nops, size_t data) {
     return pad(sizeof(dev::WinHeader)) + pad(nops * sizeof(dev::WinStage)) + pad(nops * sizeof(dev::WinOp)) + data;
 }
 
+int nslots(const zb_backend *h) { return h->slots ? h->slots : (h->prec == 64 ? 3 : 4); }
+
+template <typename T, int NS>
+void launch_window(zb_backend *h, bool dense, unsigned grid, const uint32_t *blob, const T *tab, uint64_t tiles) {
+    const dim3 g(grid), b(dev::WinShape<NS>::kLanes);
+    if (dense && h->prefetch)
+        hipLaunchKernelGGL((dev::k_window<T, NS, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+    else if (dense)
+        hipLaunchKernelGGL((dev::k_window<T, NS, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+    else if (h->prefetch)
+        hipLaunchKernelGGL((dev::k_window<T, NS, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+    else
+        hipLaunchKernelGGL((dev::k_window<T, NS, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+}
+
 template <typename T>
 void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint64_t need, const double *src,
              size_t n_complex) {
@@ -302,11 +318,15 @@ template <typename T>
 void flush_T(zb_backend *h) {
     if (h->win_ops.size() == 1) {  // one op: the per-op kernel is the cheaper pass
         const zb_backend::POp op = h->win_ops[0];
-        const T *d = reinterpret_cast<const T *>(op.kind == dev::kWDiag ? h->win_tables.data() : h->win_data.data()) +
-                     op.data_off;
-        const size_t len = op.kind == dev::kWDiag ? (1u << op.k) : (1u << (2 * op.k));
-        std::vector<double> inter(2 * len);
-        for (size_t i = 0; i < len; ++i) inter[2 * i] = (double)d[i], inter[2 * i + 1] = (double)d[len + i];
+        std::vector<double> inter;
+        if (op.kind != dev::kWCx) {  // a CX carries no matrix
+            const T *d =
+                reinterpret_cast<const T *>(op.kind == dev::kWDiag ? h->win_tables.data() : h->win_data.data()) +
+                op.data_off;
+            const size_t len = op.kind == dev::kWDiag ? (1u << op.k) : (1u << (2 * op.k));
+            inter.resize(2 * len);
+            for (size_t i = 0; i < len; ++i) inter[2 * i] = (double)d[i], inter[2 * i + 1] = (double)d[len + i];
+        }
         h->win_ops.clear();
         h->win_data.clear();
         h->win_tables.clear();
@@ -340,9 +360,10 @@ void flush_T(zb_backend *h) {
     uint32_t cur = 0;
     size_t begin = 0;
     bool dense = false;
+    const int ns = nslots(h);
     auto close = [&](size_t end) {
         uint32_t sl = cur;
-        for (uint32_t b = 0; b < m && __builtin_popcount(sl) < dev::kSlots; ++b) sl |= 1u << b;
+        for (uint32_t b = 0; b < m && __builtin_popcount(sl) < ns; ++b) sl |= 1u << b;
         dev::WinStage st{};
         int i = 0;
         for (uint32_t b = 0; b < m; ++b)
@@ -354,7 +375,7 @@ void flush_T(zb_backend *h) {
     };
     for (size_t i = 0; i < h->win_ops.size(); ++i) {
         const uint32_t nb = needs(h->win_ops[i]);
-        if (__builtin_popcount(cur | nb) > dev::kSlots) {
+        if (__builtin_popcount(cur | nb) > ns) {
             close(i);
             begin = i;
             cur = 0;
@@ -363,6 +384,7 @@ void flush_T(zb_backend *h) {
     }
     close(h->win_ops.size());
     hdr.nstages = (uint32_t)stages.size();
+    hdr.nslots = (uint32_t)ns;
     std::vector<dev::WinOp> wops(h->win_ops.size());
     for (size_t si = 0; si < stages.size(); ++si) {
         const uint32_t sl = stage_slots[si];
@@ -423,15 +445,8 @@ void flush_T(zb_backend *h) {
     const uint64_t tiles = h->len >> m;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, h->win_grid);
     h->timed(ZB_K_WINDOW, h->sweep_bytes(), [&] {
-        const dim3 g(grid), b(dev::kBlock);
-        if (dense && h->prefetch)
-            hipLaunchKernelGGL((dev::k_window<T, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
-        else if (dense)
-            hipLaunchKernelGGL((dev::k_window<T, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
-        else if (h->prefetch)
-            hipLaunchKernelGGL((dev::k_window<T, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
-        else
-            hipLaunchKernelGGL((dev::k_window<T, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), dblob, dtab, tiles);
+        if (ns == 3) launch_window<T, 3>(h, dense, grid, dblob, dtab, tiles);
+        else launch_window<T, 4>(h, dense, grid, dblob, dtab, tiles);
     });
     check(hipGetLastError(), "k_window");
     h->win_ops.clear();
@@ -447,7 +462,7 @@ void flush(zb_backend *h) {
 
 template <typename T>
 void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
-    if (!tiled(h) || k > (uint32_t)dev::kSlots) {
+    if (!tiled(h) || k > (uint32_t)nslots(h)) {
         flush(h);
         return exact_unitary_T<T>(h, qs, k, mat);
     }
@@ -825,6 +840,7 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
         flush(h);
         if (k == "prefetch") h->prefetch = value != 0;
         else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
+        else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
         else throw std::runtime_error("zb_tune: unknown key or bad value: " + k);
     });
 }

@@ -241,19 +241,18 @@ __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__restrict_
 // (|Q| = 12, Q sorted; local bit i <-> qubit Q[i]; Q holds the lowest
 // qubits, so a tile's HBM accesses are 128-byte runs).  The tile is staged
 // in LDS with one pad element per 16 (bank spread).  The window is cut into
-// stages: a stage fixes 4 local "slot" bits; each of the 256 lanes gathers
-// the 16 amplitudes that differ only in those bits into registers, applies
-// the stage's ops there with exactly the per-op kernels' arithmetic, and
-// scatters them back.  Ops admitted to a window: 1q dense and dense k <= 4
-// whose qubits are slots of one stage; diagonals on any qubits
-// (elementwise: bits outside Q are fixed per tile); CX whose target is a
-// slot (its control may be a slot, another local bit, or fixed per tile).
-// Window descriptors are uniform and read with scalar loads.
+// stages: a stage fixes NS local "slot" bits; each lane gathers the 2^NS
+// amplitudes that differ only in those bits into registers, applies the
+// stage's ops there with exactly the per-op kernels' arithmetic, and
+// scatters them back.  A workgroup has 2^(12-NS) lanes.  Ops admitted to a
+// window: 1q dense and dense k <= NS whose qubits are slots of one stage;
+// diagonals on any qubits (elementwise: bits outside Q are fixed per tile);
+// CX whose target is a slot (its control may be a slot, another local bit,
+// or fixed per tile).  Window descriptors are uniform (scalar loads).
 constexpr int kTileBits = 12;
 constexpr int kTile = 1 << kTileBits;
 constexpr int kTilePad = kTile + (kTile >> 4);
-constexpr int kSlots = 4;
-constexpr int kRegs = 1 << kSlots;
+constexpr int kMaxSlots = 4;
 constexpr uint32_t kFixedBit = 0x80000000u;  // table bit outside Q: qubit in the low bits
 constexpr int kDescBytes = 65536;           // window descriptor blob limit
 
@@ -271,24 +270,24 @@ struct WinOp {
 };
 
 struct WinStage {
-    uint32_t slot[kSlots];  // ascending local bit positions
+    uint32_t slot[kMaxSlots];  // ascending local bit positions (first NS used)
     uint32_t op_begin, op_end;
 };
 
 struct WinHeader {
     uint32_t m, nstages, nops, desc_words;
-    uint32_t stage_off, op_off, data_off, pad;  // word offsets inside the blob
+    uint32_t stage_off, op_off, data_off, nslots;  // word offsets inside the blob
     uint32_t q[kTileBits];
 };
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
 
-template <typename T, int S>
+template <typename T, int NS, int S>
 __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
     const T m00r = d[0], m01r = d[1], m10r = d[2], m11r = d[3];
     const T m00i = d[4], m01i = d[5], m10i = d[6], m11i = d[7];
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) {
+    for (int r = 0; r < (1 << NS); ++r) {
         if ((r >> S) & 1) continue;
         const int a = r, c = r | (1 << S);
         const T ar = xr[a], ai = xi[a], br = xr[c], bi = xi[c];
@@ -300,7 +299,7 @@ __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
 }
 
 // Dense op on the slots of mask M (matrix local bit b <-> b-th set bit).
-template <typename T, int M>
+template <typename T, int NS, int M>
 __device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d) {
     constexpr int K = __builtin_popcount(M);
     constexpr int D = 1 << K;
@@ -308,11 +307,11 @@ __device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d)
     {
         int b = 0;
 #pragma unroll
-        for (int s = 0; s < kSlots; ++s)
+        for (int s = 0; s < NS; ++s)
             if ((M >> s) & 1) pos[b++] = s;
     }
 #pragma unroll
-    for (int g = 0; g < kRegs; ++g) {
+    for (int g = 0; g < (1 << NS); ++g) {
         if (g & M) continue;  // g enumerates the register bits outside M
         T yr[D], yi[D], vr[D], vi[D];
 #pragma unroll
@@ -348,10 +347,10 @@ __device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d)
     }
 }
 
-template <typename T, int ST, int SC>
+template <typename T, int NS, int ST, int SC>
 __device__ __forceinline__ void reg_cx_slot(T *xr, T *xi) {
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) {
+    for (int r = 0; r < (1 << NS); ++r) {
         if (((r >> ST) & 1) || !((r >> SC) & 1)) continue;
         const int c = r | (1 << ST);
         const T t0 = xr[r], t1 = xi[r];
@@ -362,10 +361,10 @@ __device__ __forceinline__ void reg_cx_slot(T *xr, T *xi) {
     }
 }
 
-template <typename T, int ST>
+template <typename T, int NS, int ST>
 __device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
 #pragma unroll
-    for (int r = 0; r < kRegs; ++r) {
+    for (int r = 0; r < (1 << NS); ++r) {
         if ((r >> ST) & 1) continue;
         const int c = r | (1 << ST);
         const T a0 = xr[r], a1 = xi[r], b0 = xr[c], b1 = xi[c];
@@ -376,33 +375,48 @@ __device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
     }
 }
 
-template <typename T>
+template <typename T, int NS, int M>
+__device__ __forceinline__ void reg_dense_if(T *xr, T *xi, const T *__restrict__ d) {
+    if constexpr (M < (1 << NS)) reg_dense<T, NS, M>(xr, xi, d);
+}
+
+template <typename T, int NS>
 __device__ __forceinline__ void reg_dense_dispatch(T *xr, T *xi, uint32_t mask, const T *__restrict__ d) {
     switch (mask) {
-        case 0x3: reg_dense<T, 0x3>(xr, xi, d); break;
-        case 0x5: reg_dense<T, 0x5>(xr, xi, d); break;
-        case 0x6: reg_dense<T, 0x6>(xr, xi, d); break;
-        case 0x9: reg_dense<T, 0x9>(xr, xi, d); break;
-        case 0xA: reg_dense<T, 0xA>(xr, xi, d); break;
-        case 0xC: reg_dense<T, 0xC>(xr, xi, d); break;
-        case 0x7: reg_dense<T, 0x7>(xr, xi, d); break;
-        case 0xB: reg_dense<T, 0xB>(xr, xi, d); break;
-        case 0xD: reg_dense<T, 0xD>(xr, xi, d); break;
-        case 0xE: reg_dense<T, 0xE>(xr, xi, d); break;
-        case 0xF: reg_dense<T, 0xF>(xr, xi, d); break;
+        case 0x3: reg_dense_if<T, NS, 0x3>(xr, xi, d); break;
+        case 0x5: reg_dense_if<T, NS, 0x5>(xr, xi, d); break;
+        case 0x6: reg_dense_if<T, NS, 0x6>(xr, xi, d); break;
+        case 0x7: reg_dense_if<T, NS, 0x7>(xr, xi, d); break;
+        case 0x9: reg_dense_if<T, NS, 0x9>(xr, xi, d); break;
+        case 0xA: reg_dense_if<T, NS, 0xA>(xr, xi, d); break;
+        case 0xC: reg_dense_if<T, NS, 0xC>(xr, xi, d); break;
+        case 0xB: reg_dense_if<T, NS, 0xB>(xr, xi, d); break;
+        case 0xD: reg_dense_if<T, NS, 0xD>(xr, xi, d); break;
+        case 0xE: reg_dense_if<T, NS, 0xE>(xr, xi, d); break;
+        case 0xF: reg_dense_if<T, NS, 0xF>(xr, xi, d); break;
     }
 }
+
+// Workgroup size of the tiled kernel for NS slots, and the occupancy asked
+// of the compiler (workgroups per CU; the f64 tile uses 68 KB of LDS, two
+// per CU).
+template <int NS>
+struct WinShape {
+    static constexpr int kLanes = kTile >> NS;
+    static constexpr int kRegs = 1 << NS;
+};
 
 // kDense: the window holds dense k >= 2 ops; kPrefetch: the next tile's HBM
 // loads go to registers while the current tile computes (separate instances
 // keep the register budget of the common 1q/CX/diagonal windows low).
 // Global addressing: row j of a tile starts at the uniform element offset
-// base | hi(j); a lane adds its 32-bit offset lo (its low 8 local bits),
-// which needs every Q[0..7] < 31 - log2(sizeof(T)) (the host checks).
-template <typename T, bool kDense, bool kPrefetch>
-__global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__restrict__ im,
-                                                   const uint32_t *__restrict__ blob, const T *__restrict__ tables,
-                                                   uint64_t tiles) {
+// base | hi(j); a lane adds its offset lo (its low local bits).
+template <typename T, int NS, bool kDense, bool kPrefetch>
+__global__ __launch_bounds__(WinShape<NS>::kLanes, 2) void k_window(T *__restrict__ re, T *__restrict__ im,
+                                                                   const uint32_t *__restrict__ blob,
+                                                                   const T *__restrict__ tables, uint64_t tiles) {
+    constexpr int B = WinShape<NS>::kLanes, R = WinShape<NS>::kRegs;
+    constexpr int LB = kTileBits - NS;  // local bits carried by the lane index
     __shared__ T sre[kTilePad], sim[kTilePad];
     const int t = threadIdx.x;
     const WinHeader *hdr = reinterpret_cast<const WinHeader *>(blob);
@@ -410,13 +424,11 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
     const WinOp *ops = reinterpret_cast<const WinOp *>(blob + hdr->op_off);
     const T *mats = reinterpret_cast<const T *>(blob + hdr->data_off);
     const int nst = (int)hdr->nstages;
-    uint32_t lo = 0;  // the lane's element offset within a row
+    uint64_t lo = 0;  // the lane's element offset within a row
 #pragma unroll
-    for (int i = 0; i < 8; ++i)
-        if ((t >> i) & 1) lo |= 1u << hdr->q[i];
-    const uint32_t lob = lo * (uint32_t)sizeof(T);  // < 2^32 (see above)
-    auto at = [&](T *p, uint64_t h) -> T & { return *reinterpret_cast<T *>(reinterpret_cast<char *>(p + h) + lob); };
-    const int lt = padded(t);  // LDS slot of local index t (+ 272*j for row j)
+    for (int i = 0; i < LB; ++i)
+        if ((t >> i) & 1) lo |= 1ull << hdr->q[i];
+    const int lt = padded(t);  // LDS slot of local index t (+ (B + B/16)*j for row j)
     auto tile_base = [&](uint64_t tile) {
         uint64_t b = tile;
 #pragma unroll
@@ -426,34 +438,34 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
     auto row = [&](uint64_t base, int j) {
         uint64_t h = base;
 #pragma unroll
-        for (int i = 8; i < kTileBits; ++i)
-            if ((j >> (i - 8)) & 1) h |= 1ull << hdr->q[i];
+        for (int i = LB; i < kTileBits; ++i)
+            if ((j >> (i - LB)) & 1) h |= 1ull << hdr->q[i];
         return h;
     };
-    T pr[kRegs], pi[kRegs];
+    T pr[R], pi[R];
     uint64_t tile = blockIdx.x;
     uint64_t base = tile < tiles ? tile_base(tile) : 0;
     if (kPrefetch && tile < tiles) {
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) {
-            const uint64_t h = row(base, j);
-            pr[j] = at(re, h);
-            pi[j] = at(im, h);
+        for (int j = 0; j < R; ++j) {
+            const uint64_t g = row(base, j) | lo;
+            pr[j] = re[g];
+            pi[j] = im[g];
         }
     }
     while (tile < tiles) {
         if constexpr (!kPrefetch) {
 #pragma unroll
-            for (int j = 0; j < kRegs; ++j) {
-                const uint64_t h = row(base, j);
-                pr[j] = at(re, h);
-                pi[j] = at(im, h);
+            for (int j = 0; j < R; ++j) {
+                const uint64_t g = row(base, j) | lo;
+                pr[j] = re[g];
+                pi[j] = im[g];
             }
         }
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) {
-            sre[lt + (kBlock + kBlock / 16) * j] = pr[j];
-            sim[lt + (kBlock + kBlock / 16) * j] = pi[j];
+        for (int j = 0; j < R; ++j) {
+            sre[lt + (B + B / 16) * j] = pr[j];
+            sim[lt + (B + B / 16) * j] = pi[j];
         }
         __syncthreads();
         const uint64_t next = tile + gridDim.x;
@@ -461,24 +473,30 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
         if constexpr (kPrefetch) {
             if (next < tiles) {
 #pragma unroll
-                for (int j = 0; j < kRegs; ++j) {
-                    const uint64_t h = row(nbase, j);
-                    pr[j] = at(re, h);
-                    pi[j] = at(im, h);
+                for (int j = 0; j < R; ++j) {
+                    const uint64_t g = row(nbase, j) | lo;
+                    pr[j] = re[g];
+                    pi[j] = im[g];
                 }
             }
         }
         for (int s = 0; s < nst; ++s) {
             const WinStage *st = stages + s;
-            const int s0 = (int)st->slot[0], s1 = (int)st->slot[1], s2 = (int)st->slot[2], s3 = (int)st->slot[3];
-            const int lb = (int)insert_zero(insert_zero(insert_zero(insert_zero((uint64_t)t, s0), s1), s2), s3);
+            int sl[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) sl[i] = (int)st->slot[i];
+            int lb = t;
+#pragma unroll
+            for (int i = 0; i < NS; ++i) lb = (int)insert_zero((uint64_t)lb, sl[i]);
             const int plb = padded(lb);
-            T xr[kRegs], xi[kRegs];
+            T xr[R], xi[R];
             // lb and the register bits are disjoint, so padded(lb | c) =
             // padded(lb) + padded(c): a lane base plus a uniform offset
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) {
-                const int c = ((r & 1) << s0) | (((r >> 1) & 1) << s1) | (((r >> 2) & 1) << s2) | (((r >> 3) & 1) << s3);
+            for (int r = 0; r < R; ++r) {
+                int c = 0;
+#pragma unroll
+                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
                 xr[r] = sre[plb + padded(c)];
                 xi[r] = sim[plb + padded(c)];
             }
@@ -490,35 +508,41 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
                     case kW1q: {
                         const T *d = mats + op->data_off;
                         switch (sel) {
-                            case 0: reg_1q<T, 0>(xr, xi, d); break;
-                            case 1: reg_1q<T, 1>(xr, xi, d); break;
-                            case 2: reg_1q<T, 2>(xr, xi, d); break;
-                            case 3: reg_1q<T, 3>(xr, xi, d); break;
+                            case 0: reg_1q<T, NS, 0>(xr, xi, d); break;
+                            case 1: reg_1q<T, NS, 1>(xr, xi, d); break;
+                            case 2: if constexpr (NS > 2) reg_1q<T, NS, 2>(xr, xi, d); break;
+                            case 3: if constexpr (NS > 3) reg_1q<T, NS, 3>(xr, xi, d); break;
                         }
                         break;
                     }
                     case kWDense:
-                        if constexpr (kDense) reg_dense_dispatch<T>(xr, xi, sel, mats + op->data_off);
+                        if constexpr (kDense) reg_dense_dispatch<T, NS>(xr, xi, sel, mats + op->data_off);
                         break;
                     case kWDiag: {
                         // table index: bits fixed by the tile or by the lane, plus the
-                        // bits that vary across the lane's 16 registers
+                        // bits that vary across the lane's registers
                         const T *d = tables + op->data_off;
                         const int k = (int)op->k, D = 1 << k;
-                        uint32_t jt = 0, j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+                        uint32_t jt = 0, js[NS];
+#pragma unroll
+                        for (int i = 0; i < NS; ++i) js[i] = 0;
                         for (int b = 0; b < k; ++b) {
                             const uint32_t bb = op->bits[b];
-                            if (bb & kFixedBit) jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
-                            else if ((int)bb == s0) j0 = 1u << b;
-                            else if ((int)bb == s1) j1 = 1u << b;
-                            else if ((int)bb == s2) j2 = 1u << b;
-                            else if ((int)bb == s3) j3 = 1u << b;
-                            else jt |= (uint32_t)((lb >> bb) & 1) << b;
+                            if (bb & kFixedBit) {
+                                jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
+                                continue;
+                            }
+                            bool slot = false;
+#pragma unroll
+                            for (int i = 0; i < NS; ++i)
+                                if ((int)bb == sl[i]) js[i] = 1u << b, slot = true;
+                            if (!slot) jt |= (uint32_t)((lb >> bb) & 1) << b;
                         }
 #pragma unroll
-                        for (int r = 0; r < kRegs; ++r) {
-                            const uint32_t j =
-                                jt | ((r & 1) ? j0 : 0) | ((r & 2) ? j1 : 0) | ((r & 4) ? j2 : 0) | ((r & 8) ? j3 : 0);
+                        for (int r = 0; r < R; ++r) {
+                            uint32_t j = jt;
+#pragma unroll
+                            for (int i = 0; i < NS; ++i) j |= ((r >> i) & 1) ? js[i] : 0u;
                             const T wr = d[j], wi = d[D + j];
                             const T ar = xr[r], ai = xi[r];
                             xr[r] = wr * ar - wi * ai;
@@ -530,27 +554,27 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
                         const uint32_t st_ = sel & 3, sc = (sel >> 2) & 3, mode = sel >> 4;
                         if (mode == kCtrlSlot) {
                             switch (st_ * 4 + sc) {
-                                case 1: reg_cx_slot<T, 0, 1>(xr, xi); break;
-                                case 2: reg_cx_slot<T, 0, 2>(xr, xi); break;
-                                case 3: reg_cx_slot<T, 0, 3>(xr, xi); break;
-                                case 4: reg_cx_slot<T, 1, 0>(xr, xi); break;
-                                case 6: reg_cx_slot<T, 1, 2>(xr, xi); break;
-                                case 7: reg_cx_slot<T, 1, 3>(xr, xi); break;
-                                case 8: reg_cx_slot<T, 2, 0>(xr, xi); break;
-                                case 9: reg_cx_slot<T, 2, 1>(xr, xi); break;
-                                case 11: reg_cx_slot<T, 2, 3>(xr, xi); break;
-                                case 12: reg_cx_slot<T, 3, 0>(xr, xi); break;
-                                case 13: reg_cx_slot<T, 3, 1>(xr, xi); break;
-                                case 14: reg_cx_slot<T, 3, 2>(xr, xi); break;
+                                case 1: reg_cx_slot<T, NS, 0, 1>(xr, xi); break;
+                                case 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 0, 2>(xr, xi); break;
+                                case 3: if constexpr (NS > 3) reg_cx_slot<T, NS, 0, 3>(xr, xi); break;
+                                case 4: reg_cx_slot<T, NS, 1, 0>(xr, xi); break;
+                                case 6: if constexpr (NS > 2) reg_cx_slot<T, NS, 1, 2>(xr, xi); break;
+                                case 7: if constexpr (NS > 3) reg_cx_slot<T, NS, 1, 3>(xr, xi); break;
+                                case 8: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 0>(xr, xi); break;
+                                case 9: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 1>(xr, xi); break;
+                                case 11: if constexpr (NS > 3) reg_cx_slot<T, NS, 2, 3>(xr, xi); break;
+                                case 12: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 0>(xr, xi); break;
+                                case 13: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 1>(xr, xi); break;
+                                case 14: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 2>(xr, xi); break;
                             }
                         } else {
                             const uint32_t aux = op->aux;
                             const bool on = mode == kCtrlLocal ? ((lb >> aux) & 1) : ((base >> aux) & 1);
                             switch (st_) {
-                                case 0: reg_cx_all<T, 0>(xr, xi, on); break;
-                                case 1: reg_cx_all<T, 1>(xr, xi, on); break;
-                                case 2: reg_cx_all<T, 2>(xr, xi, on); break;
-                                case 3: reg_cx_all<T, 3>(xr, xi, on); break;
+                                case 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
+                                case 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
+                                case 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
+                                case 3: if constexpr (NS > 3) reg_cx_all<T, NS, 3>(xr, xi, on); break;
                             }
                         }
                         break;
@@ -558,19 +582,25 @@ __global__ __launch_bounds__(kBlock) void k_window(T *__restrict__ re, T *__rest
                 }
             }
 #pragma unroll
-            for (int r = 0; r < kRegs; ++r) {
-                const int c = ((r & 1) << s0) | (((r >> 1) & 1) << s1) | (((r >> 2) & 1) << s2) | (((r >> 3) & 1) << s3);
+            for (int r = 0; r < R; ++r) {
+                int c = 0;
+#pragma unroll
+                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
                 sre[plb + padded(c)] = xr[r];
                 sim[plb + padded(c)] = xi[r];
             }
             __syncthreads();
         }
 #pragma unroll
-        for (int j = 0; j < kRegs; ++j) {
-            const T vr = sre[lt + (kBlock + kBlock / 16) * j], vi = sim[lt + (kBlock + kBlock / 16) * j];
-            const uint64_t h = row(base, j);
-            at(re, h) = vr;
-            at(im, h) = vi;
+        for (int j = 0; j < R; ++j) {
+            pr[j] = sre[lt + (B + B / 16) * j];
+            pi[j] = sim[lt + (B + B / 16) * j];
+        }
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint64_t g = row(base, j) | lo;
+            re[g] = pr[j];
+            im[g] = pi[j];
         }
         __syncthreads();
         tile = next;
