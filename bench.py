@@ -41,7 +41,7 @@ def effective_bytes(ops, n, s):
 TUNE = {}
 
 
-def run_gpu(n, comp, prec, mode, steps, warmup, torch):
+def run_gpu(n, comp, prec, mode, steps, warmup):
     """Time `steps` full circuits (zero init + every op + finish) after
     `warmup` untimed ones; kernel durations come from HIP events the library
     records around each launch on its own stream."""
@@ -53,14 +53,13 @@ def run_gpu(n, comp, prec, mode, steps, warmup, torch):
     step_times, recs = [], []
     for step in range(warmup + steps):
         timed = step >= warmup
-        torch.cuda.synchronize()
+        be.finish()
         if timed:
             be.profile_begin()
         t0 = time.perf_counter()
         be.reset_all()
         be.apply_compiled(comp)
         be.finish()
-        torch.cuda.synchronize()
         dt = time.perf_counter() - t0
         if timed:
             step_times.append(dt)
@@ -122,7 +121,7 @@ def main():
     ap.add_argument("--also", default="", help="extra fusion:mode pairs to time once, e.g. 5:0,5:1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
-                    help="also run a plain torch copy of 2^n f64 (re) + 2^n f64 (im), the known-size kernel "
+                    help="also run a plain device copy of 2^n f64 (re) + 2^n f64 (im), the known-size kernel "
                     "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--tune", default="", help="tiled-kernel knobs, e.g. prefetch=1,win_grid=1024")
     args = ap.parse_args()
@@ -137,11 +136,9 @@ def main():
         )
         sys.exit(2)
 
-    import torch
-
     import zeno_amd as z
 
-    if not torch.cuda.is_available():
+    if z.device_count() == 0:
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
 
@@ -154,17 +151,12 @@ def main():
     _log(f"compiled {comp.stats['input_gates']} gates -> {len(ops)} ops at fusion {args.fusion} in {t_compile:.2f} s")
 
     if args.calibrate_copy:
-        src = torch.ones(2 << n, dtype=torch.float64, device="cuda")
-        dst = torch.empty_like(src)
-        for _ in range(2):
-            dst.copy_(src)
-        torch.cuda.synchronize()
-        _log(f"calibration copy: {2 * src.numel() * 8} bytes moved per copy ({src.numel() * 8} read, same written)")
-        del src, dst
-        torch.cuda.empty_cache()
+        ms = z.calibration_copy(2 << n, reps=2)
+        nb = (2 << n) * 8
+        _log(f"calibration copy (k_copy): {2 * nb} bytes moved per copy ({nb} read, same written), {ms:.3f} ms")
 
-    be64, s64, cls64, bytes64 = run_gpu(n, comp, 64, args.mode, args.steps, args.warmup, torch)
-    be32, s32, cls32, bytes32 = run_gpu(n, comp, 32, args.mode, args.steps, args.warmup, torch)
+    be64, s64, cls64, bytes64 = run_gpu(n, comp, 64, args.mode, args.steps, args.warmup)
+    be32, s32, cls32, bytes32 = run_gpu(n, comp, 32, args.mode, args.steps, args.warmup)
     err = be32.max_abs_diff(be64)
     norm64 = be64.norm_sqr()
     be32.close()
@@ -175,7 +167,7 @@ def main():
         fk, md = (int(v) for v in spec.split(":"))
         c2 = z.compile(prog, n, fusion=fk)
         for prec in (64, 32):
-            b2, t2, k2, by2 = run_gpu(n, c2, prec, md, 1, 1, torch)
+            b2, t2, k2, by2 = run_gpu(n, c2, prec, md, 1, 1)
             b2.close()
             alternatives.append({"fusion": fk, "mode": "tiled" if md == 1 else "exact", "precision": f"f{prec}",
                                  "ops": len(c2), "s_per_circuit": round(t2, 4), "hbm_gbps": round(by2 / t2 / 1e9, 2),

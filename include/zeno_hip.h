@@ -135,6 +135,17 @@ void *zb_stream(zb_backend *h);
 /* max_i |a_i - b_i| over two states of equal size (any precisions), f64,
  * computed on the device; used to state the f32 error against f64. */
 int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out);
+/* Number of visible HIP devices (0 when there is none or the runtime
+ * cannot start); never fails. */
+int zb_device_count(void);
+/* Free and total device memory in bytes: the HBM counterpart of the
+ * reference's RAM budget query (src/mem.rs:11-17). */
+int zb_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
+/* Runs a plain device copy of n_doubles f64 values `reps` times and
+ * returns the average milliseconds per copy: the known-size kernel
+ * (8 B read + 8 B written per element) that scales hardware byte
+ * counters.  Allocates and frees its own buffers. */
+int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms);
 /* Tuning knobs of the tiled kernel (flushes first): "prefetch" (0/1:
  * issue the next tile's HBM loads into registers while the current tile
  * computes) and "win_grid" (workgroups per window launch). */
@@ -215,6 +226,19 @@ int64_t zr_run(const zr_compiled *c, int precision, uint64_t shots,
                uint64_t seed, const zr_noise *noise, int mode,
                uint64_t *keys, uint64_t *counts, int64_t cap,
                double *sim_seconds);
+
+/* Batched noisy / dynamic trajectories (src/exec.rs:459-619,
+ * src/noise.rs:91-192): every shot runs in one persistent kernel launch
+ * captured in a hipGraph, each workgroup executing whole trajectories on
+ * its own state.  Shot s uses Xoshiro256++ seeded with
+ * splitmix64(seed ^ s) and the frozen draw order, so out[s] (the shot's
+ * clbit word) equals the CPU per-shot path's.  Needs n <= 20 and
+ * unitary/diagonal ops on <= 3 qubits (zr_trajectories_supported);
+ * kernel_seconds gets the launch-to-completion time. */
+int zr_trajectories_supported(const zr_compiled *c);
+int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots,
+                        uint64_t seed, const zr_noise *noise, uint64_t *out,
+                        double *kernel_seconds);
 
 /* Seeded circuit generators (src/circuit.rs:138-174 and the benchmark
  * configurations' matching/RZZ variant).  Return the instruction count;

@@ -107,6 +107,25 @@ def test_random_circuits_bit_exact(orc, prec, mode):
                 _assert_same(g, o, f"n={n} fusion={fusion}")
 
 
+@pytest.mark.parametrize("tune", [{"win_grid": 3, "prefetch": 1}, {"win_grid": 5, "prefetch": 0},
+                                  {"win_grid": 2, "slots": 4}, {"win_grid": 4, "slots": 3}])
+@pytest.mark.parametrize("prec", [64, 32])
+def test_tiled_multi_tile_workgroups_bit_exact(orc, prec, tune):
+    """A small window grid makes every workgroup walk many tiles (the
+    prefetch and write-back hand-over between consecutive tiles)."""
+    for n in (12, 14, 17):
+        for prog in (orc.gen_random(n, 10, 77 + n), orc.gen_matching(n, 10, 5 + n, 3)):
+            for fusion in (1, 3):
+                _say(f"multi-tile n={n} fusion={fusion} prec={prec} tune={tune}")
+                o = orc.State(n, prec)
+                o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+                with z.HipBackend(n, prec, z.HipBackend.TILED) as g:
+                    for k, v in tune.items():
+                        g.tune(k, v)
+                    g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                    _assert_same(g, o, f"n={n} fusion={fusion} tune={tune}")
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_adversarial_circuits_bit_exact(orc, mode):
     cases = [(random_mixed(orc, n, 80, 9000 + n), n) for n in (3, 6, 10, 13, 15)]
@@ -161,24 +180,69 @@ def test_c1_counts_identical(orc):
         assert r.counts.total() == 1024 and r.backend == "hip-f64"
 
 
-def test_dynamic_and_noisy_runs_match_oracle(orc):
+@pytest.mark.parametrize("mode", MODES)
+def test_dynamic_and_noisy_runs_match_oracle(orc, mode):
+    """Per-shot path: EXACT replays every shot through the zb_* calls,
+    TILED runs all shots as one batched trajectory launch."""
     I = orc.instr
     rows = [I("x", [0]), I("h", [1]), I("cx", [1, 2]), I("cx", [0, 1]), I("h", [0]),
             orc.measure(0, 0), orc.measure(1, 1), I("x", [2], cond=(1, 1)), I("z", [2], cond=(0, 1)),
             orc.measure(2, 2)]
     prog = orc.program(rows)
-    _say("teleport per-shot")
+    _say(f"teleport per-shot mode={mode}")
     want, _ = orc.run(orc.Compiled(prog, 3, cregs=[1, 1, 1]), 200, 5)
-    got = z.run_program(prog, 3, cregs=[1, 1, 1], opts=z.RunOptions(shots=200, seed=5))
+    got = z.run_program(prog, 3, cregs=[1, 1, 1], opts=z.RunOptions(shots=200, seed=5, mode=mode))
     assert got.raw_counts == want
     nz = {"depolarizing_1q": 0.02, "depolarizing_2q": 0.05, "amplitude_damping": 0.03,
           "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
     full = orc.program([orc.gen_random(5, 4, 9), orc.measure_all(5)])
     for prec in (64, 32):
-        _say(f"noisy per-shot prec={prec}")
+        _say(f"noisy per-shot prec={prec} mode={mode}")
         want, _ = orc.run(orc.Compiled(full, 5, fusion=0), 64, 17, prec=prec, noise=nz)
-        got = z.run_program(full, 5, opts=z.RunOptions(shots=64, seed=17, precision=prec, noise=z.NoiseModel(**nz)))
+        got = z.run_program(full, 5, opts=z.RunOptions(shots=64, seed=17, precision=prec, noise=z.NoiseModel(**nz),
+                                                        mode=mode))
         assert got.raw_counts == want
+
+
+def _traj_counts(words):
+    vals, cnt = np.unique(np.asarray(words, np.uint64), return_counts=True)
+    return {int(v): int(c) for v, c in zip(vals, cnt)}
+
+
+@pytest.mark.parametrize("prec", [32, 64])
+def test_batched_trajectories_match_oracle(orc, prec):
+    """Batched noisy trajectories (every noise channel, reset, conditionals,
+    3-qubit gates) give the CPU trajectory sampler's counts exactly."""
+    I = orc.instr
+    nz = {"depolarizing_1q": 0.01, "depolarizing_2q": 0.03, "bit_flip": 0.01, "phase_flip": 0.01,
+          "amplitude_damping": 0.02, "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.03}
+    rows = [orc.gen_random(9, 6, 4), I("ccx", [0, 4, 8]), I("cswap", [1, 2, 7]), orc.reset(3),
+            orc.measure(5, 5), I("rx", [3], [0.7], cond=(1, 1)), orc.barrier(), orc.measure_all(9)]
+    prog = orc.program(rows)
+    for fusion in (0, 3):
+        c = orc.Compiled(prog, 9, cregs=[5, 4], fusion=fusion)
+        want, _ = orc.run(c, 300, 123, prec=prec, noise=nz)
+        zc = z.compile(prog, 9, cregs=[5, 4], fusion=fusion)
+        assert z.trajectories_supported(zc)
+        _say(f"batched trajectories 9q prec={prec} fusion={fusion}")
+        words, _ = z.run_trajectories(zc, 300, 123, prec, z.NoiseModel(**nz))
+        assert _traj_counts(words) == want
+        # shot s of a run seeded S is the single shot of a run seeded S ^ s
+        for shot in range(0, 300, 7):
+            one, _ = orc.run(c, 1, 123 ^ shot, prec=prec, noise=nz)
+            assert list(one) == [int(words[shot])], shot
+    # the configuration of the noisy-trajectory benchmark, 16 qubits depth 30
+    nz16 = {"depolarizing_1q": 1e-3, "depolarizing_2q": 1e-2, "amplitude_damping": 5e-3,
+            "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+    prog16 = orc.program([orc.gen_random(16, 30, 2024), orc.measure_all(16)])
+    orc.set_threads(16)
+    try:
+        want, _ = orc.run(orc.Compiled(prog16, 16, fusion=0), 48, 99, prec=prec, noise=nz16)
+    finally:
+        orc.set_threads(1)
+    _say(f"batched trajectories 16q d30 prec={prec}")
+    words, _ = z.run_trajectories(z.compile(prog16, 16, fusion=0), 48, 99, prec, z.NoiseModel(**nz16))
+    assert _traj_counts(words) == want
 
 
 def _adjoint_roundtrip(n, prec, prog, fusion, tol):
@@ -221,3 +285,44 @@ def test_qft24_closed_form():
             rev |= ((j >> b) & 1) << (n - 1 - b)
         want = np.exp(2j * np.pi * ((x * j) % (1 << n)) / (1 << n)) / math.sqrt(1 << n)
         assert np.max(np.abs(sv[rev] - want)) < 1e-10
+
+
+def _run_pair(orc, prog, n, fusion, prec, shots, seed):
+    """GPU run vs oracle run of the same compiled program; state bits and
+    counts must be identical."""
+    full = orc.program([prog, orc.measure_all(n)])
+    oc = orc.Compiled(full, n, fusion=fusion)
+    o = orc.State(n, prec)
+    o.apply_compiled(oc)
+    gc = z.compile(full, n, fusion=fusion)
+    with z.HipBackend(n, prec) as g:
+        g.apply_compiled(gc)
+        _assert_same(g, o, f"n={n} fusion={fusion} prec={prec}")
+        assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed))
+
+
+def test_c2_qft24_and_random26_bit_exact_with_1m_shots(orc):
+    """QFT on 24 qubits after a seeded U3 layer, and the
+    random circuit at 26 qubits depth 12, fusion K in {2, 3, 4}, f64, 1M shots."""
+    orc.set_threads(16)
+    try:
+        for fusion in (2, 3, 4):
+            _say(f"C2 qft24 fusion={fusion}")
+            _run_pair(orc, orc.gen_u3_qft(24, 11), 24, fusion, 64, 1_000_000, 42)
+        for fusion in (2, 3, 4):
+            _say(f"C2 random26 fusion={fusion}")
+            _run_pair(orc, orc.gen_random(26, 12, 26), 26, fusion, 64, 1_000_000, 7)
+    finally:
+        orc.set_threads(1)
+
+
+def test_c3_33_qubit_state_roundtrip():
+    """BASELINE config C3's large state: 33 qubits (128 GiB f64, 64 GiB f32),
+    matching circuit with RZZ ladders, forward then adjoint returns |0...0>."""
+    free, _ = z.device_memory()
+    n = 33
+    for prec, tol in ((64, 1e-9), (32, 1e-4)):
+        if free < (1 << n) * (16 if prec == 64 else 8) + (8 << 30):
+            pytest.skip("not enough device memory for a 33-qubit state")
+        _say(f"C3 33q roundtrip prec={prec}")
+        _adjoint_roundtrip(n, prec, z.matching_circuit(n, 5, 3, 5).to_program()[0], 1, tol)

@@ -6,7 +6,16 @@ side, C ABI in include/zeno_hip.h).  Importing this package does not touch
 the GPU; constructing a backend or running a circuit does, and fails loudly
 if the library or a HIP device is missing.
 """
-from .backend import DeviceError, HipBackend, MemoryBudget, Unsupported, ZenoError
+from .backend import (
+    DeviceError,
+    HipBackend,
+    MemoryBudget,
+    Unsupported,
+    ZenoError,
+    calibration_copy,
+    device_count,
+    device_memory,
+)
 from .circuit import Circuit, matching_circuit, qft, random_circuit, u3_qft
 from .exec import (
     Compiled,
@@ -22,13 +31,15 @@ from .exec import (
     plan,
     run_compiled,
     run_program,
+    run_trajectories,
     state_bytes,
+    trajectories_supported,
 )
 from . import ir
 
 __all__ = [
-    "Circuit", "Compiled", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
+    "Circuit", "Compiled", "calibration_copy", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
     "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
     "format_key", "ir", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
-    "run_compiled", "run_program", "state_bytes", "u3_qft",
+    "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",
 ]

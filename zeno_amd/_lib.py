@@ -54,6 +54,11 @@ SIGNATURES = {
     "zr_compiled_final_measures": (I64, [P, P]),
     "zr_apply_ops": (I32, [P, P, I64, I64]),
     "zr_run": (I64, [P, I32, U64, U64, P, I32, P, P, I64, P]),
+    "zb_device_count": (I32, []),
+    "zb_mem_info": (I32, [P, P]),
+    "zb_calibration_copy": (I32, [U64, I32, P]),
+    "zr_trajectories_supported": (I32, [P]),
+    "zr_run_trajectories": (I32, [P, I32, U64, U64, P, P, P]),
     "zr_gen_random": (I64, [U32, U32, U64, P, I64]),
     "zr_gen_qft": (I64, [U32, P, I64]),
     "zr_gen_u3_qft": (I64, [U32, U64, P, I64]),

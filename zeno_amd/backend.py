@@ -48,6 +48,26 @@ def _check(rc: int, what: str) -> int:
     return rc
 
 
+def device_count() -> int:
+    """Visible HIP devices (0 without a GPU)."""
+    return int(lib().zb_device_count())
+
+
+def device_memory():
+    """(free, total) device memory in bytes."""
+    f, t = C.c_uint64(), C.c_uint64()
+    _check(lib().zb_mem_info(C.byref(f), C.byref(t)), "mem_info")
+    return int(f.value), int(t.value)
+
+
+def calibration_copy(n_doubles: int, reps: int = 3) -> float:
+    """Average ms of a plain f64 device copy of n_doubles elements (8 B read
+    and 8 B written each)."""
+    ms = C.c_double()
+    _check(lib().zb_calibration_copy(n_doubles, reps, C.byref(ms)), "calibration_copy")
+    return float(ms.value)
+
+
 class HipBackend:
     """make_backend(n, precision, BackendChoice::Hip) (src/exec.rs:306-331)."""
 

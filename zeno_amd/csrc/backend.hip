@@ -618,8 +618,7 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
         set_error("the HIP backend supports 1..35 qubits (got " + std::to_string(n_qubits) + ")");
         return ZB_ERR_UNSUPPORTED;
     }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+    if (zb_device_count() == 0) {
         set_error("no HIP device is available (the hip backend needs an AMD GPU)");
         return ZB_ERR_UNSUPPORTED;
     }
@@ -832,6 +831,59 @@ int zb_finish(zb_backend *h) {
         flush(h);
         h->sync();
     });
+}
+
+int zb_device_count(void) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess) return 0;
+    return ndev;
+}
+
+int zb_mem_info(uint64_t *free_bytes, uint64_t *total_bytes) {
+    if (zb_device_count() == 0) {
+        set_error("no HIP device is available (the hip backend needs an AMD GPU)");
+        return ZB_ERR_UNSUPPORTED;
+    }
+    return guarded([&] {
+        size_t f = 0, t = 0;
+        check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+        *free_bytes = f;
+        *total_bytes = t;
+    });
+}
+
+int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms) {
+    if (zb_device_count() == 0) {
+        set_error("no HIP device is available (the hip backend needs an AMD GPU)");
+        return ZB_ERR_UNSUPPORTED;
+    }
+    double *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int rc = guarded([&] {
+        check(hipMalloc(&src, n_doubles * 8), "hipMalloc");
+        check(hipMalloc(&dst, n_doubles * 8), "hipMalloc");
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        const unsigned grid = (unsigned)std::min<uint64_t>((n_doubles + dev::kBlock - 1) / dev::kBlock, 1u << 16);
+        hipLaunchKernelGGL(dev::k_fill, dim3(grid), dim3(dev::kBlock), 0, st, src, n_doubles, 1.0);
+        check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < reps; ++r)
+            hipLaunchKernelGGL(dev::k_copy, dim3(grid), dim3(dev::kBlock), 0, st, dst, src, n_doubles);
+        check(hipGetLastError(), "k_copy");
+        check(hipEventRecord(e1, st), "hipEventRecord");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        float f = 0.f;
+        check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+        if (ms) *ms = reps > 0 ? (double)f / reps : 0.0;
+    });
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
 }
 
 int zb_tune(zb_backend *h, const char *key, int64_t value) {

@@ -15,10 +15,6 @@
 
 using namespace zeno;
 
-struct zr_compiled {
-    Compiled c;
-};
-
 namespace {
 
 struct BackendDeleter {
@@ -240,6 +236,11 @@ int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t se
                     raw[cl] += kv.second;
                 }
             }
+        } else if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && zr_trajectories_supported(cc)) {
+            // run_per_shot (exec.rs:459-549) as one batched trajectory launch
+            std::vector<uint64_t> words(shots);
+            ok(zr_run_trajectories(cc, precision, shots, seed, nz, words.data(), nullptr));
+            for (uint64_t w : words) ++raw[w];
         } else if (c.dynamic || !c.final_measures.empty()) {  // run_per_shot (exec.rs:459-549)
             for (uint64_t shot = 0; shot < shots; ++shot) {
                 ok(zb_reset_all(h.get()));

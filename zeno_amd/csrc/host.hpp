@@ -82,6 +82,15 @@ int gate_lookup(const char *name, int *arity, int *n_params);
 Compiled compile(const zr_instr *prog, int64_t n, uint32_t n_qubits, const uint32_t *creg_sizes,
                  uint32_t n_cregs, int fusion_max, int diag_max);
 
+}  // namespace zeno
+
+// Opaque handle of the C ABI (include/zeno_hip.h).
+struct zr_compiled {
+    zeno::Compiled c;
+};
+
+namespace zeno {
+
 // Thread-local last-error plumbing for the C ABI.
 void set_error(const std::string &msg);
 const char *last_error();

@@ -221,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_state(const T *__restrict__ r
     if (threadIdx.x == 0) out[blockIdx.x] = tot;
 }
 
-__global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__restrict__ in, uint64_t n,
+static __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__restrict__ in, uint64_t n,
                                                        double *__restrict__ out) {
     __shared__ double s[kBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kReduceSpan;
@@ -591,21 +591,35 @@ __global__ __launch_bounds__(WinShape<NS>::kLanes, 2) void k_window(T *__restric
             }
             __syncthreads();
         }
+        // write-back through its own registers: pr/pi may already hold the
+        // prefetched next tile
+        T wr[R], wi[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            pr[j] = sre[lt + (B + B / 16) * j];
-            pi[j] = sim[lt + (B + B / 16) * j];
+            wr[j] = sre[lt + (B + B / 16) * j];
+            wi[j] = sim[lt + (B + B / 16) * j];
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             const uint64_t g = row(base, j) | lo;
-            re[g] = pr[j];
-            im[g] = pi[j];
+            re[g] = wr[j];
+            im[g] = wi[j];
         }
         __syncthreads();
         tile = next;
         base = nbase;
     }
+}
+
+// Plain streaming copy of known size (8 B read + 8 B written per element):
+// the reference kernel that scales hardware byte counters.
+static __global__ __launch_bounds__(kBlock) void k_copy(double *__restrict__ dst, const double *__restrict__ src,
+                                                        uint64_t n) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) dst[i] = src[i];
+}
+
+static __global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ dst, uint64_t n, double v) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
 }
 
 // max_i |a_i - b_i| (complex modulus, f64) between two states of possibly

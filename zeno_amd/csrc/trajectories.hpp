@@ -1,0 +1,393 @@
+// trajectories.hpp — batched noisy trajectories (src/exec.rs:459-619,
+// src/noise.rs:91-192) in one persistent launch.
+//
+// Each workgroup runs whole trajectories, one after another (shot =
+// blockIdx.x, += gridDim.x), on its own split-complex state in HBM.  The
+// per-shot RNG is Xoshiro256++ seeded with splitmix64(seed ^ shot), drawn in
+// the reference's frozen order (docs/NOISE.md:150-166); it is uniform across
+// the workgroup, so every lane computes the same draws.  Sweeps use exactly
+// the per-op kernels' arithmetic and prob_one uses the fixed reduction order
+// of the rest of the backend, so every measurement decision — hence every
+// shot's clbit word — equals the CPU path's for the same seed.
+//
+// Every helper is force-inlined: the kernel then makes no device-function
+// calls, needs no scratch, and keeps 4 (f64) / 6 (f32) waves per SIMD.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace zeno {
+namespace dev {
+
+constexpr int kTrajMaxK = 3;
+
+struct TrajOp {
+    uint32_t kind;  // 0 unitary, 1 diagonal, 2 cx, 3 measure, 4 reset
+    uint32_t k;     // qubits of a unitary/diagonal (1..3)
+    uint32_t q[kTrajMaxK];  // sorted qubits; cx: control, target; measure/reset: qubit
+    uint32_t clbit;
+    uint32_t is_if, creg_off, creg_len;
+    uint32_t data_off;  // matrix (2*4^k T) or diagonal (2*2^k T) in the data block
+    uint64_t value;
+};
+
+struct TrajNoise {
+    double dep1, dep2, bit_flip, phase_flip, damping, ro01, ro10;
+};
+
+struct DevRng {
+    uint64_t s0, s1, s2, s3;
+    __device__ static uint64_t mix(uint64_t x) {
+        uint64_t z = x + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    __device__ void seed(uint64_t x) {  // rand_xoshiro seed_from_u64
+        s0 = mix(x);
+        s1 = mix(x + 0x9E3779B97F4A7C15ull);
+        s2 = mix(x + 2 * 0x9E3779B97F4A7C15ull);
+        s3 = mix(x + 3 * 0x9E3779B97F4A7C15ull);
+    }
+    __device__ uint64_t next() {
+        const uint64_t r = ((s0 + s3) << 23 | (s0 + s3) >> 41) + s0;
+        const uint64_t t = s1 << 17;
+        s2 ^= s0;
+        s3 ^= s1;
+        s1 ^= s2;
+        s0 ^= s3;
+        s2 ^= t;
+        s3 = s3 << 45 | s3 >> 19;
+        return r;
+    }
+    __device__ double f64() { return (double)(next() >> 11) * 0x1.0p-53; }
+    __device__ uint64_t range(uint64_t lo, uint64_t hi) {  // rand 0.8 UniformInt<u64>
+        const uint64_t r = hi - lo;
+        const uint64_t zone = (r << __builtin_clzll(r)) - 1;
+        while (true) {
+            const uint64_t v = next();
+            const uint64_t mlo = v * r, mhi = __umul64hi(v, r);
+            if (mlo <= zone) return lo + mhi;
+        }
+    }
+};
+
+template <typename T>
+struct TrajCtx {
+    T *re, *im;
+    uint64_t len;
+    double *red;     // LDS: 8 x 256 lane partials + 256 block sums
+    const T *pauli;  // LDS: X then Y as [re 4 | im 4]
+};
+
+template <typename T>
+__device__ __forceinline__ void t_1q(const TrajCtx<T> c, uint32_t q, const T *m) {
+    const T m00r = m[0], m01r = m[1], m10r = m[2], m11r = m[3];
+    const T m00i = m[4], m01i = m[5], m10i = m[6], m11i = m[7];
+    const uint64_t s = 1ull << q;
+    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
+        const uint64_t a = insert_zero(h, q), b = a + s;
+        const T ar = c.re[a], ai = c.im[a], br = c.re[b], bi = c.im[b];
+        c.re[a] = m00r * ar - m00i * ai + m01r * br - m01i * bi;
+        c.im[a] = m00r * ai + m00i * ar + m01r * bi + m01i * br;
+        c.re[b] = m10r * ar - m10i * ai + m11r * br - m11i * bi;
+        c.im[b] = m10r * ai + m10i * ar + m11r * bi + m11i * br;
+    }
+    __syncthreads();
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void t_kq(const TrajCtx<T> c, const uint32_t *qs, const T *m) {
+    constexpr int D = 1 << K;
+    for (uint64_t g = threadIdx.x; g < (c.len >> K); g += blockDim.x) {
+        uint64_t base = g;
+#pragma unroll
+        for (int b = 0; b < K; ++b) base = insert_zero(base, qs[b]);
+        T xr[D], xi[D];
+#pragma unroll
+        for (int j = 0; j < D; ++j) {
+            uint64_t o = base;
+#pragma unroll
+            for (int b = 0; b < K; ++b)
+                if ((j >> b) & 1) o |= 1ull << qs[b];
+            xr[j] = c.re[o];
+            xi[j] = c.im[o];
+        }
+        // Rows stay a runtime loop so the LDS matrix is re-read per group
+        // instead of being hoisted into registers.
+#pragma unroll 1
+        for (int r = 0; r < D; ++r) {
+            T ar = T(0), ai = T(0);
+#pragma unroll
+            for (int cc = 0; cc < D; ++cc) {
+                const T wr = m[r * D + cc], wi = m[D * D + r * D + cc];
+                ar = ar + wr * xr[cc] - wi * xi[cc];
+                ai = ai + wr * xi[cc] + wi * xr[cc];
+            }
+            uint64_t o = base;
+#pragma unroll
+            for (int b = 0; b < K; ++b)
+                if ((r >> b) & 1) o |= 1ull << qs[b];
+            c.re[o] = ar;
+            c.im[o] = ai;
+        }
+    }
+    __syncthreads();
+}
+
+// kern_diag1_scalar (src/state.rs:498-527): d0 on the a side, d1 on the b side.
+template <typename T>
+__device__ __forceinline__ void t_diag1(const TrajCtx<T> c, uint32_t q, T d0r, T d1r, T d0i, T d1i) {
+    const uint64_t s = 1ull << q;
+    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
+        const uint64_t a = insert_zero(h, q), b = a + s;
+        const T ar = c.re[a], ai = c.im[a], br = c.re[b], bi = c.im[b];
+        c.re[a] = d0r * ar - d0i * ai;
+        c.im[a] = d0r * ai + d0i * ar;
+        c.re[b] = d1r * br - d1i * bi;
+        c.im[b] = d1r * bi + d1i * br;
+    }
+    __syncthreads();
+}
+
+// Diagonal op; d = [re D | im D] (k == 1: d0r, d1r, d0i, d1i).
+template <typename T>
+__device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, int k, const T *d) {
+    const int D = 1 << k;
+    if (k == 1) {
+        t_diag1(c, qs[0], d[0], d[1], d[2], d[3]);
+        return;
+    } else {
+        for (uint64_t i = threadIdx.x; i < c.len; i += blockDim.x) {
+            uint32_t j = 0;
+#pragma unroll
+            for (int b = 0; b < kTrajMaxK; ++b)
+                if (b < k) j |= (uint32_t)((i >> qs[b]) & 1) << b;
+            const T wr = d[j], wi = d[D + j], ar = c.re[i], ai = c.im[i];
+            c.re[i] = wr * ar - wi * ai;
+            c.im[i] = wr * ai + wi * ar;
+        }
+    }
+    __syncthreads();
+}
+
+template <typename T>
+__device__ __forceinline__ void t_cx(const TrajCtx<T> c, uint32_t ctl, uint32_t tgt) {
+    const uint32_t lo = ctl < tgt ? ctl : tgt, hi = ctl < tgt ? tgt : ctl;
+    for (uint64_t g = threadIdx.x; g < (c.len >> 2); g += blockDim.x) {
+        const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
+        const T r0 = c.re[a], r1 = c.re[b], i0 = c.im[a], i1 = c.im[b];
+        c.re[a] = r1;
+        c.re[b] = r0;
+        c.im[a] = i1;
+        c.im[b] = i0;
+    }
+    __syncthreads();
+}
+
+// prob_one in the backend's fixed order: 4096-element blocks (16 serial
+// adds per lane, pairwise tree over 256 lanes), then the block sums the
+// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).
+template <typename T>
+__device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, uint32_t q) {
+    const uint64_t half = c.len >> 1, qs = 1ull << q;
+    const int t = threadIdx.x;
+    const int nb = (int)((half + kReduceSpan - 1) / kReduceSpan);
+    double *s = c.red;                 // [8][256] lane partials
+    double *bsum = c.red + 8 * kBlock;  // block sums (<= 256)
+    for (int b0 = 0; b0 < nb; b0 += 8) {
+        const int nchunk = nb - b0 < 8 ? nb - b0 : 8;
+        for (int bi = 0; bi < nchunk; ++bi) {
+            double a = 0.0;
+            const uint64_t base = (uint64_t)(b0 + bi) * kReduceSpan;
+#pragma unroll
+            for (int j = 0; j < kReduceSpan / kBlock; ++j) {
+                const uint64_t e = base + (uint64_t)j * kBlock + t;
+                if (e < half) {
+                    const uint64_t idx = insert_zero(e, q) + qs;
+                    const double r = (double)c.re[idx], m = (double)c.im[idx];
+                    a = a + (r * r + m * m);
+                }
+            }
+            s[bi * kBlock + t] = a;
+        }
+        __syncthreads();
+        for (int st = kBlock / 2; st >= 1; st >>= 1) {
+            if (t < st)
+                for (int bi = 0; bi < nchunk; ++bi) s[bi * kBlock + t] = s[bi * kBlock + t] + s[bi * kBlock + t + st];
+            __syncthreads();
+        }
+        if (t < nchunk) bsum[b0 + t] = s[t * kBlock];
+        __syncthreads();
+    }
+    if (nb > 1) {
+        double a = 0.0;
+        if (t < nb) a = a + bsum[t];  // the 4096-wide pass: only j = 0 is populated
+        s[t] = a;
+        __syncthreads();
+        for (int st = kBlock / 2; st >= 1; st >>= 1) {
+            if (t < st) s[t] = s[t] + s[t + st];
+            __syncthreads();
+        }
+        const double r = s[0];
+        __syncthreads();
+        return r;
+    }
+    const double r = bsum[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ void t_collapse(const TrajCtx<T> c, uint32_t q, int outcome, T scale) {
+    const uint64_t s = 1ull << q;
+    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
+        const uint64_t a = insert_zero(h, q);
+        const uint64_t keep = outcome ? a + s : a, zero = outcome ? a : a + s;
+        c.re[keep] = c.re[keep] * scale;
+        c.im[keep] = c.im[keep] * scale;
+        c.re[zero] = T(0);
+        c.im[zero] = T(0);
+    }
+    __syncthreads();
+}
+
+// CpuBackend::measure (src/exec.rs:201-207)
+template <typename T>
+__device__ __forceinline__ int t_measure(const TrajCtx<T> c, uint32_t q, double u) {
+    const double p1 = t_prob_one(c, q);
+    const int outcome = u < p1;
+    const double p = outcome ? p1 : 1.0 - p1;
+    t_collapse(c, q, outcome, (T)(1.0 / sqrt(p)));
+    return outcome;
+}
+
+// noise.rs:91-100, through the same unitary / diagonal arithmetic the CPU
+// path uses (signed zeros included).
+template <typename T>
+__device__ __forceinline__ void t_pauli(const TrajCtx<T> c, uint32_t q, uint64_t code) {
+    if (code == 1) t_1q(c, q, c.pauli + 0);
+    else if (code == 2) t_1q(c, q, c.pauli + 8);
+    else if (code == 3) t_diag1(c, q, T(1), T(-1), T(0), T(0));
+}
+
+template <typename T>
+__device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, const TrajNoise &nz, const uint32_t *qs, int k, DevRng &rng) {
+    const double pd = k == 1 ? nz.dep1 : nz.dep2;
+    if (pd > 0.0 && rng.f64() < pd) {
+        const uint64_t idx = rng.range(1, 1ull << (2 * k));
+#pragma unroll
+        for (int b = 0; b < kTrajMaxK; ++b)
+            if (b < k) t_pauli(c, qs[b], (idx >> (2 * b)) & 3);
+    }
+#pragma unroll
+    for (int b = 0; b < kTrajMaxK; ++b) {
+        if (b >= k) break;
+        const uint32_t q = qs[b];
+        if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, q, 1);
+        if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, q, 3);
+        if (nz.damping > 0.0) {  // noise.rs:114-127
+            const double u = rng.f64();
+            const double pj = nz.damping * t_prob_one(c, q);
+            if (u < pj) {
+                t_measure(c, q, 0.0);
+                t_pauli(c, q, 1);
+            } else {
+                const double sc = 1.0 / sqrt(1.0 - pj);
+                t_diag1(c, q, (T)sc, (T)(sc * sqrt(1.0 - nz.damping)), T(0), T(0));
+            }
+        }
+    }
+}
+
+template <typename T>
+__device__ __forceinline__ int t_readout(const TrajNoise &nz, int bit, DevRng &rng) {
+    const double p = bit ? nz.ro10 : nz.ro01;
+    if (p > 0.0 && rng.f64() < p) return !bit;
+    return bit;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uint32_t n, const TrajOp *__restrict__ ops,
+                                                         uint32_t nops, const uint32_t *__restrict__ fm, uint32_t nfm,
+                                                         const T *__restrict__ mats, TrajNoise nz, int noisy,
+                                                         uint64_t seed, uint64_t shots, uint64_t *__restrict__ out) {
+    __shared__ double red[9 * kBlock];
+    __shared__ T smat[2 << (2 * kTrajMaxK)];
+    __shared__ T spauli[16];
+    if (threadIdx.x < 16) {
+        // X = [[0,1],[1,0]], Y = [[0,-i],[i,0]], each as [re 4 | im 4]
+        const int t = threadIdx.x;
+        spauli[t] = (t == 1 || t == 2 || t == 14) ? T(1) : t == 13 ? T(-1) : T(0);
+    }
+    __syncthreads();
+    const uint64_t len = 1ull << n;
+    TrajCtx<T> c{ws + (uint64_t)blockIdx.x * 2 * len, ws + (uint64_t)blockIdx.x * 2 * len + len, len, red, spauli};
+    for (uint64_t shot = blockIdx.x; shot < shots; shot += gridDim.x) {
+        for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) {
+            c.re[i] = i == 0 ? T(1) : T(0);
+            c.im[i] = T(0);
+        }
+        __syncthreads();
+        DevRng rng;
+        rng.seed(DevRng::mix(seed ^ shot));
+        uint64_t clbits = 0;
+        for (uint32_t oi = 0; oi < nops; ++oi) {
+            const TrajOp &op = ops[oi];
+            if (op.is_if) {
+                const uint64_t mask = op.creg_len >= 64 ? ~0ull : ((1ull << op.creg_len) - 1);
+                if (((clbits >> op.creg_off) & mask) != op.value) continue;
+            }
+            const uint32_t kind = op.kind, k = op.k;
+            const uint32_t qs[kTrajMaxK] = {op.q[0], op.q[1], op.q[2]};
+            if (kind <= 1) {  // stage the matrix / diagonal in LDS
+                const uint32_t cnt = kind == 0 ? 2u << (2 * k) : 2u << k;
+                for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) smat[i] = mats[op.data_off + i];
+                __syncthreads();
+            }
+            switch (kind) {
+                case 0:
+                    if (k == 1) t_1q(c, qs[0], smat);
+                    else if (k == 2) t_kq<T, 2>(c, qs, smat);
+                    else t_kq<T, 3>(c, qs, smat);
+                    if (noisy) t_gate_noise(c, nz, qs, (int)k, rng);
+                    break;
+                case 1:
+                    t_diag(c, qs, (int)k, smat);
+                    if (noisy) t_gate_noise(c, nz, qs, (int)k, rng);
+                    break;
+                case 2: {
+                    t_cx(c, qs[0], qs[1]);
+                    if (noisy) {
+                        const uint32_t sq[kTrajMaxK] = {qs[0] < qs[1] ? qs[0] : qs[1], qs[0] < qs[1] ? qs[1] : qs[0], 0};
+                        t_gate_noise(c, nz, sq, 2, rng);
+                    }
+                    break;
+                }
+                case 3: {
+                    int bit = t_measure(c, qs[0], rng.f64());
+                    if (noisy) bit = t_readout<T>(nz, bit, rng);
+                    clbits = (clbits & ~(1ull << op.clbit)) | ((uint64_t)bit << op.clbit);
+                    break;
+                }
+                case 4:
+                    if (t_measure(c, qs[0], rng.f64())) t_pauli(c, qs[0], 1);
+                    break;
+            }
+        }
+        for (uint32_t f = 0; f < nfm; ++f) {
+            int bit = t_measure(c, fm[2 * f], rng.f64());
+            if (noisy) bit = t_readout<T>(nz, bit, rng);
+            clbits = (clbits & ~(1ull << fm[2 * f + 1])) | ((uint64_t)bit << fm[2 * f + 1]);
+        }
+        if (threadIdx.x == 0) out[shot] = clbits;
+        __syncthreads();
+    }
+}
+
+}  // namespace dev
+}  // namespace zeno

@@ -183,14 +183,11 @@ def device_budget_bytes(fraction: float = 0.9) -> int:
             return int(env)
         except ValueError:
             pass
-    try:
-        import torch
+    from .backend import device_count, device_memory
 
-        if torch.cuda.is_available():
-            free, _total = torch.cuda.mem_get_info()
-            return int(free * fraction)
-    except Exception:
-        pass
+    if device_count() > 0:
+        free, _total = device_memory()
+        return int(free * fraction)
     return 288 << 30
 
 
@@ -296,6 +293,28 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
         seed=seed, sim_time=secs, statevector=statevector, stats=c.stats,
         mem_bytes=state_bytes(c.n_qubits, prec), notices=notices,
     )
+
+
+def trajectories_supported(c: Compiled) -> bool:
+    """Whether `run_trajectories` takes the circuit: <= 20 qubits and
+    unitary/diagonal ops on <= 3 qubits."""
+    return bool(lib().zr_trajectories_supported(c.handle))
+
+
+def run_trajectories(c: Compiled, shots: int, seed: int, precision: int = 64, noise: Optional[NoiseModel] = None):
+    """Every shot of run_per_shot (src/exec.rs:459-549, src/noise.rs:91-192)
+    in one persistent kernel launch.  Returns (clbit word per shot as
+    uint64, kernel seconds); shot s equals the CPU per-shot path's shot s."""
+    if noise is not None:
+        noise.validate()
+    out = np.zeros(max(shots, 1), np.uint64)
+    secs = C.c_double(0.0)
+    cn = _CNoise(**dataclasses.asdict(noise)) if noise is not None and not noise.is_trivial() else None
+    rc = lib().zr_run_trajectories(
+        c.handle, precision, shots, seed, C.byref(cn) if cn is not None else None, ptr(out), C.byref(secs)
+    )
+    _check(rc, "run_trajectories")
+    return out[:shots], secs.value
 
 
 def _splitmix64(x: int) -> int:
