@@ -107,6 +107,94 @@ def cpu_baseline(args, prog):
     }
 
 
+NOISE_C4 = {"depolarizing_1q": 1e-3, "depolarizing_2q": 1e-2, "amplitude_damping": 5e-3,
+            "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+
+
+def bench_trajectories(args):
+    """Batched noisy trajectories: brickwork random circuit 16q depth 30,
+    4096 seeded trajectories, f32 states, one persistent launch; the CPU
+    trajectory sampler timed on a bounded sample of the same shots."""
+    import zeno_amd as z
+
+    n, depth, shots = 16, 30, args.traj_shots
+    prog = z.random_circuit(n, depth, 2024).to_program()[0]
+    prog = z.Circuit(n).extend(prog).measure_all().to_program()[0]
+    c = z.compile(prog, n, fusion=0)  # noisy runs compile with fusion 0 (src/lib.rs:228-231)
+    nz = z.NoiseModel(**NOISE_C4)
+    z.run_trajectories(c, 64, 1, 32, nz)
+    _, secs = z.run_trajectories(c, shots, args.seed, 32, nz)
+    out = {"workload": f"random_circuit {n}q depth {depth} (brickwork), fusion 0, {shots} trajectories, f32, "
+                       f"noise {NOISE_C4}",
+           "ops": len(c), "s": round(secs, 4), "trajectories_per_s": round(shots / secs, 1)}
+    if not args.no_cpu:
+        import oracle
+
+        m = args.traj_cpu_shots
+        oc = oracle.Compiled(prog, n, fusion=0)
+        t0 = time.perf_counter()
+        oracle.run(oc, m, args.seed, prec=32, noise=NOISE_C4)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(m / dt, 2), "unit": "trajectories/s", "cores": 1, "kind": "port",
+                               "sample": f"first {m} of the {shots} trajectories, oracle/zeno_oracle.c per-shot "
+                                         f"sampler, f32, 1 thread"}
+    return out
+
+
+def bench_serve(args):
+    """Mixed-size serving: a Poisson stream of seeded random circuits (n ~
+    U{10..28}, depth ~ U{8..40}, 1000 shots, f32 or f64 each) through the
+    continuous-batching server, against the same stream served one at a
+    time (one slot)."""
+    import threading
+
+    import numpy as np
+
+    import zeno_amd as z
+
+    rng = np.random.default_rng(args.seed)
+    reqs = []
+    for i in range(args.serve_requests):
+        n = int(rng.integers(10, 29))
+        depth = int(rng.integers(8, 41))
+        prec = 32 if rng.integers(0, 2) else 64
+        prog = z.Circuit(n).extend(z.random_circuit(n, depth, 1000 + i).to_program()[0]).measure_all().to_program()[0]
+        reqs.append((z.compile(prog, n, fusion=1), prec, 1000, 5000 + i))
+    results = {}
+    for rate in [float(x) for x in args.serve_rates.split(",") if x]:
+        gaps = rng.exponential(1.0 / rate, len(reqs))
+        arrivals = np.cumsum(gaps) - gaps[0]
+        row = {}
+        for label, slots in (("served", args.serve_slots), ("one_at_a_time", 1)):
+            with z.Server(args.serve_pool << 30, slots) as srv:
+                ids = [None] * len(reqs)
+
+                def feed():
+                    t0 = time.perf_counter()
+                    for k, (c, prec, shots, seed) in enumerate(reqs):
+                        dt = arrivals[k] - (time.perf_counter() - t0)
+                        if dt > 0:
+                            time.sleep(dt)
+                        ids[k] = srv.submit(c, prec, shots, seed)
+
+                th = threading.Thread(target=feed)
+                th.start()
+                th.join()
+                done = [srv.wait(i) for i in ids]
+            lat = np.array([d.latency for d in done]) * 1e3
+            span = max(d.t_end for d in done) - min(d.t_submit for d in done)
+            row[label] = {"slots": slots, "circuits_per_s": round(len(done) / span, 2),
+                          "p50_ms": round(float(np.percentile(lat, 50)), 2),
+                          "p99_ms": round(float(np.percentile(lat, 99)), 2),
+                          "mean_queue_ms": round(float(np.mean([d.queued for d in done])) * 1e3, 2)}
+            _log(f"serve rate={rate} {label}: {row[label]}")
+        results[str(rate)] = row
+    return {"workload": f"{len(reqs)} requests, Poisson arrivals, n ~ U{{10..28}}, depth ~ U{{8..40}}, "
+                        f"1000 shots, f32/f64 at random, fusion 1 (compiled before arrival), pool "
+                        f"{args.serve_pool} GiB",
+            "rates": results}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -124,6 +212,13 @@ def main():
                     help="also run a plain device copy of 2^n f64 (re) + 2^n f64 (im), the known-size kernel "
                     "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--tune", default="", help="tiled-kernel knobs, e.g. prefetch=1,win_grid=1024")
+    ap.add_argument("--traj-shots", type=int, default=4096)
+    ap.add_argument("--traj-cpu-shots", type=int, default=24)
+    ap.add_argument("--serve-requests", type=int, default=80)
+    ap.add_argument("--serve-rates", default="50", help="Poisson arrival rates (req/s), comma separated")
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-pool", type=int, default=48, help="serving pool in GiB")
+    ap.add_argument("--no-extra", action="store_true", help="skip the trajectory and serving legs")
     args = ap.parse_args()
     for kv in [x for x in args.tune.split(",") if x]:
         k, v = kv.split("=")
@@ -237,6 +332,13 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_extra:
+        _log("batched trajectories ...")
+        out["trajectories"] = bench_trajectories(args)
+        _log("serving ...")
+        out["serve"] = bench_serve(args)
+        first = next(iter(out["serve"]["rates"].values()))
+        out["p99_latency_ms_mixed_serve"] = first["served"]["p99_ms"]
     print(json.dumps(out), flush=True)
 
 

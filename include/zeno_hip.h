@@ -61,6 +61,18 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out);
 
 /* Drop for Box<dyn Backend>: frees device memory. */
 void zb_destroy(zb_backend *h);
+/* make_backend over caller-owned device memory: re and im each hold 2^n
+ * values of the precision (8 or 4 bytes), stay owned by the caller and
+ * must outlive the handle; the state is set to |0...0>.  For callers that
+ * sub-allocate states from one HBM pool (the serving loop). */
+int zb_create_bound(uint32_t n_qubits, int precision, void *re, void *im,
+                    zb_backend **out);
+/* Re-point a handle at other caller-owned state memory, possibly with a
+ * different qubit count and precision (finishes queued work first, frees
+ * a state the handle owned, sets |0...0>); the stream, parameter arena and
+ * scratch buffers are kept. */
+int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re,
+              void *im);
 
 /* Backend::name (src/exec.rs:135): "hip-f64" or "hip-f32". */
 const char *zb_name(const zb_backend *h);
@@ -239,6 +251,36 @@ int zr_trajectories_supported(const zr_compiled *c);
 int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots,
                         uint64_t seed, const zr_noise *noise, uint64_t *out,
                         double *kernel_seconds);
+
+/* ---------------------------------------------------------------------
+ * Layer 3: serving loop (no counterpart in the reference, which runs one
+ * circuit per invocation with mem::plan admission, src/mem.rs:79-130)
+ * ------------------------------------------------------------------- */
+
+/* Continuous batching on one GPU: one HBM pool of pool_bytes (0 = 80% of
+ * free device memory) is allocated at start; `slots` worker slots each own
+ * a backend with its own HIP stream.  A request is admitted, first fit in
+ * arrival order, when a slot is idle and the pool has room for its state
+ * (2 * 2^n * 4 or 8 bytes); its state lives in the pool until it
+ * completes.  mode: see zb_set_mode. */
+typedef struct zs_server zs_server;
+int zs_create(uint64_t pool_bytes, int slots, int mode, zs_server **out);
+/* Queue a run of a compiled circuit (the server copies it; the caller may
+ * free it at once).  Returns the request id (> 0), or < 0 when the state
+ * can never fit in the pool (ZB_ERR_MEMORY) or the arguments are bad. */
+int64_t zs_submit(zs_server *s, const zr_compiled *c, int precision,
+                  uint64_t shots, uint64_t seed, const zr_noise *noise);
+/* Block until request `id` completes; writes its clbit-word counts as
+ * zr_run does and times[3] = submit, start, end (seconds since the server
+ * started).  Each id can be waited for once. */
+int64_t zs_wait(zs_server *s, int64_t id, uint64_t *keys, uint64_t *counts,
+                int64_t cap, double *times);
+/* out[4] = queued requests, running requests, largest free pool block,
+ * pool size (bytes). */
+void zs_status(zs_server *s, int64_t *out);
+/* Fails queued requests, waits for running ones, frees the pool.  No
+ * zs_wait may still be blocked on this server. */
+void zs_destroy(zs_server *s);
 
 /* Seeded circuit generators (src/circuit.rs:138-174 and the benchmark
  * configurations' matching/RZZ variant).  Return the instruction count;

@@ -326,3 +326,68 @@ def test_c3_33_qubit_state_roundtrip():
             pytest.skip("not enough device memory for a 33-qubit state")
         _say(f"C3 33q roundtrip prec={prec}")
         _adjoint_roundtrip(n, prec, z.matching_circuit(n, 5, 3, 5).to_program()[0], 1, tol)
+
+
+def _serve_cases(orc):
+    """Mixed requests: sizes 10..24 qubits, both precisions, a dynamic and a
+    noisy circuit among them."""
+    cases = []
+    rng = np.random.default_rng(5)
+    for i in range(22):
+        n = int(rng.integers(10, 25))
+        depth = int(rng.integers(4, 16))
+        prog = orc.program([orc.gen_random(n, depth, 300 + i), orc.measure_all(n)])
+        cases.append((prog, n, None, 32 if i % 2 else 64, 1000, 77 + i, None))
+    I = orc.instr
+    tele = orc.program([I("x", [0]), I("h", [1]), I("cx", [1, 2]), I("cx", [0, 1]), I("h", [0]),
+                        orc.measure(0, 0), orc.measure(1, 1), I("x", [2], cond=(1, 1)), I("z", [2], cond=(0, 1)),
+                        orc.measure(2, 2)])
+    cases.append((tele, 3, [1, 1, 1], 64, 500, 5, None))
+    nz = {"depolarizing_1q": 0.01, "depolarizing_2q": 0.02, "amplitude_damping": 0.01,
+          "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+    cases.append((orc.program([orc.gen_random(8, 5, 9), orc.measure_all(8)]), 8, None, 32, 400, 11, nz))
+    return cases
+
+
+def test_serving_results_same_alone_and_in_mix(orc):
+    """Each served result equals the same request run alone (and, for the
+    small ones, the oracle), with requests overlapping on the GPU and
+    admitted as pool space frees up."""
+    cases = _serve_cases(orc)
+    alone = []
+    for prog, n, cregs, prec, shots, seed, nz in cases:
+        c = z.compile(prog, n, cregs, fusion=0 if nz else 1)
+        r = z.run_compiled(c, z.RunOptions(shots=shots, seed=seed, precision=prec,
+                                           noise=z.NoiseModel(**nz) if nz else None))
+        alone.append(r.raw_counts)
+        if n <= 14:
+            want, _ = orc.run(orc.Compiled(prog, n, cregs=cregs, fusion=0 if nz else 1), shots, seed, prec=prec,
+                              noise=nz)
+            assert r.raw_counts == want, (n, prec, seed)
+    # a pool that holds a few 24q f64 states at a time forces queueing
+    for pool, slots in ((1 << 30, 4), (3 << 30, 8)):
+        _say(f"serving mix pool={pool >> 20} MiB slots={slots}")
+        with z.Server(pool, slots) as srv:
+            ids = []
+            for prog, n, cregs, prec, shots, seed, nz in cases:
+                c = z.compile(prog, n, cregs, fusion=0 if nz else 1)
+                ids.append(srv.submit(c, prec, shots, seed, z.NoiseModel(**nz) if nz else None))
+            got = [srv.wait(i) for i in ids]
+            assert srv.status()["queued"] == 0 and srv.status()["running"] == 0
+        for g, a in zip(got, alone):
+            assert g.raw_counts == a
+            assert g.t_submit <= g.t_start <= g.t_end
+        overlap = sum(1 for a in got for b in got if a.id < b.id and a.t_start < b.t_end and b.t_start < a.t_end)
+        assert overlap > 0, "no two requests ran at the same time"
+
+
+def test_serving_admission_limits():
+    with z.Server(64 << 20, 2) as srv:
+        big = z.compile(z.random_circuit(23, 2, 1).to_program()[0], 23)
+        with pytest.raises(z.MemoryBudget):
+            srv.submit(big, 64, 10, 1)  # 128 MiB state > 64 MiB pool
+        small = z.compile(z.random_circuit(21, 3, 1).to_program()[0], 21)
+        ids = [srv.submit(small, 64, 0, k) for k in range(3)]  # 32 MiB each: at most 2 resident
+        for i in ids:
+            r = srv.wait(i)
+            assert r.raw_counts == {}

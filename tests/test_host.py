@@ -34,6 +34,10 @@ def test_backend_fails_loudly_without_device():
         pytest.skip("a GPU is present")
     with pytest.raises(z.Unsupported):
         z.HipBackend(4)
+    with pytest.raises(z.Unsupported):
+        z.Server(1 << 30)
+    with pytest.raises(z.Unsupported):
+        z.device_memory()
 
 
 def test_generators_match_oracle(orc):

@@ -36,10 +36,11 @@ from .exec import (
     trajectories_supported,
 )
 from . import ir
+from .serve import Served, Server
 
 __all__ = [
     "Circuit", "Compiled", "calibration_copy", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
     "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
     "format_key", "ir", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
-    "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",
+    "Served", "Server", "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",
 ]

@@ -57,6 +57,13 @@ SIGNATURES = {
     "zb_device_count": (I32, []),
     "zb_mem_info": (I32, [P, P]),
     "zb_calibration_copy": (I32, [U64, I32, P]),
+    "zb_create_bound": (I32, [U32, I32, P, P, P]),
+    "zb_rebind": (I32, [P, U32, I32, P, P]),
+    "zs_create": (I32, [U64, I32, I32, P]),
+    "zs_submit": (I64, [P, P, I32, U64, U64, P]),
+    "zs_wait": (I64, [P, I64, P, P, I64, P]),
+    "zs_status": (None, [P, P]),
+    "zs_destroy": (None, [P]),
     "zr_trajectories_supported": (I32, [P]),
     "zr_run_trajectories": (I32, [P, I32, U64, U64, P, P, P]),
     "zr_gen_random": (I64, [U32, U32, U64, P, I64]),

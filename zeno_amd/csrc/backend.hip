@@ -46,6 +46,7 @@ struct zb_backend {
     uint64_t len = 0;
     size_t elem = 8;
     void *re = nullptr, *im = nullptr;
+    bool owns_state = true;  // false when bound to caller-owned device memory
     hipStream_t stream = nullptr;
     int mode = 1;
     bool prefetch = true;  // tiled kernel variant: register prefetch of the next tile
@@ -608,8 +609,9 @@ extern "C" {
 
 const char *zb_last_error(void) { return last_error(); }
 
-int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
-    *out = nullptr;
+namespace {
+
+int check_shape(uint32_t n_qubits, int precision) {
     if (precision != 32 && precision != 64) {
         set_error("precision must be 32 or 64");
         return ZB_ERR_ARG;
@@ -622,12 +624,38 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
         set_error("no HIP device is available (the hip backend needs an AMD GPU)");
         return ZB_ERR_UNSUPPORTED;
     }
-    zb_backend *h = new zb_backend();
+    return ZB_OK;
+}
+
+void set_shape(zb_backend *h, uint32_t n_qubits, int precision) {
     h->n = n_qubits;
     h->prec = precision;
     h->len = 1ull << n_qubits;
     h->elem = precision == 64 ? 8 : 4;
     h->name = precision == 64 ? "hip-f64" : "hip-f32";
+}
+
+// Stream, parameter arena and pinned result slot: everything but the state.
+void create_resources(zb_backend *h) {
+    check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+    check(hipMalloc((void **)&h->arena_dev, kArenaBytes), "hipMalloc arena");
+    check(hipHostMalloc((void **)&h->arena_host, kArenaBytes, hipHostMallocDefault), "hipHostMalloc arena");
+    check(hipHostMalloc((void **)&h->red_host, sizeof(double), hipHostMallocDefault), "hipHostMalloc red");
+}
+
+void zero_state(zb_backend *h) {
+    if (h->prec == 64) launch_zero<double>(h);
+    else launch_zero<float>(h);
+    h->sync();
+}
+
+}  // namespace
+
+int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
+    *out = nullptr;
+    if (const int rc = check_shape(n_qubits, precision); rc != ZB_OK) return rc;
+    zb_backend *h = new zb_backend();
+    set_shape(h, n_qubits, precision);
     const int rc = guarded([&] {
         size_t free_b = 0, total_b = 0;
         check(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
@@ -636,15 +664,10 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
             throw std::runtime_error("state vector for " + std::to_string(n_qubits) + " qubits at f" +
                                      std::to_string(precision) + " needs " + std::to_string(need) +
                                      " B but only " + std::to_string(free_b) + " B of device memory are free");
-        check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+        create_resources(h);
         check(hipMalloc(&h->re, h->len * h->elem), "hipMalloc re");
         check(hipMalloc(&h->im, h->len * h->elem), "hipMalloc im");
-        check(hipMalloc((void **)&h->arena_dev, kArenaBytes), "hipMalloc arena");
-        check(hipHostMalloc((void **)&h->arena_host, kArenaBytes, hipHostMallocDefault), "hipHostMalloc arena");
-        check(hipHostMalloc((void **)&h->red_host, sizeof(double), hipHostMallocDefault), "hipHostMalloc red");
-        if (precision == 64) launch_zero<double>(h);
-        else launch_zero<float>(h);
-        h->sync();
+        zero_state(h);
     });
     if (rc != ZB_OK) {
         zb_destroy(h);
@@ -654,11 +677,58 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
     return ZB_OK;
 }
 
+int zb_create_bound(uint32_t n_qubits, int precision, void *re, void *im, zb_backend **out) {
+    *out = nullptr;
+    if (const int rc = check_shape(n_qubits, precision); rc != ZB_OK) return rc;
+    if (!re || !im) {
+        set_error("zb_create_bound: state pointers must be non-null device memory");
+        return ZB_ERR_ARG;
+    }
+    zb_backend *h = new zb_backend();
+    set_shape(h, n_qubits, precision);
+    h->owns_state = false;
+    h->re = re;
+    h->im = im;
+    const int rc = guarded([&] {
+        create_resources(h);
+        zero_state(h);
+    });
+    if (rc != ZB_OK) {
+        zb_destroy(h);
+        return rc;
+    }
+    *out = h;
+    return ZB_OK;
+}
+
+int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re, void *im) {
+    if (const int rc = check_shape(n_qubits, precision); rc != ZB_OK) return rc;
+    if (!re || !im) {
+        set_error("zb_rebind: state pointers must be non-null device memory");
+        return ZB_ERR_ARG;
+    }
+    return guarded([&] {
+        h->sync();  // work queued on the old state completes first
+        h->win_ops.clear();
+        h->win_data.clear();
+        h->win_tables.clear();
+        if (h->owns_state) {
+            if (h->re) check(hipFree(h->re), "hipFree re");
+            if (h->im) check(hipFree(h->im), "hipFree im");
+        }
+        h->owns_state = false;
+        h->re = re;
+        h->im = im;
+        set_shape(h, n_qubits, precision);
+        zero_state(h);
+    });
+}
+
 void zb_destroy(zb_backend *h) {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    if (h->re) (void)hipFree(h->re);
-    if (h->im) (void)hipFree(h->im);
+    if (h->owns_state && h->re) (void)hipFree(h->re);
+    if (h->owns_state && h->im) (void)hipFree(h->im);
     if (h->arena_dev) (void)hipFree(h->arena_dev);
     if (h->arena_host) (void)hipHostFree(h->arena_host);
     if (h->red_dev) (void)hipFree(h->red_dev);

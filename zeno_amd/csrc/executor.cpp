@@ -143,6 +143,72 @@ bool valid(const zr_noise *m) {  // noise.rs:78-87
 
 }  // namespace
 
+namespace zeno {
+
+int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint64_t seed, const zr_noise *noise,
+                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out) {
+    const Compiled &c = cc->c;
+    if (!valid(noise)) {
+        set_error("invalid noise model: every probability must be in [0, 1]");
+        return ZB_ERR_ARG;
+    }
+    if (zb_n_qubits(hb) != c.n_qubits) {
+        set_error("backend and circuit qubit counts differ");
+        return ZB_ERR_ARG;
+    }
+    const int precision = zb_precision(hb);
+    const zr_noise *nz = trivial(noise) ? nullptr : noise;
+    std::unordered_map<uint64_t, uint64_t> raw;
+    try {
+        zb_backend *h = hb;
+        ok(zb_set_mode(h, mode));
+        if (!state_is_zero) ok(zb_reset_all(h));
+        if (!nz && !c.dynamic) {  // run_sampled (exec.rs:407-452)
+            for (const Op &o : c.ops) apply_op(h, o);
+            ok(zb_finish(h));
+            if (!c.final_measures.empty() && shots > 0) {
+                std::vector<uint64_t> smp(shots);
+                ok(zb_sample(h, shots, splitmix64(seed), smp.data()));
+                std::unordered_map<uint64_t, uint64_t> basis;
+                for (uint64_t s : smp) ++basis[s];
+                for (const auto &kv : basis) {
+                    uint64_t cl = 0;
+                    for (const auto &fm : c.final_measures) {
+                        const uint64_t bit = (kv.first >> fm.first) & 1;
+                        cl = (cl & ~(1ull << fm.second)) | (bit << fm.second);
+                    }
+                    raw[cl] += kv.second;
+                }
+            }
+        } else if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && zr_trajectories_supported(cc)) {
+            // run_per_shot (exec.rs:459-549) as one batched trajectory launch
+            std::vector<uint64_t> words(shots);
+            ok(zr_run_trajectories(cc, precision, shots, seed, nz, words.data(), nullptr));
+            for (uint64_t w : words) ++raw[w];
+        } else if (c.dynamic || !c.final_measures.empty()) {  // run_per_shot (exec.rs:459-549)
+            for (uint64_t shot = 0; shot < shots; ++shot) {
+                ok(zb_reset_all(h));
+                Rng rng(splitmix64(seed ^ shot));
+                uint64_t cl = 0;
+                for (const Op &o : c.ops) exec_shot_op(h, o, nz, rng, cl);
+                for (const auto &fm : c.final_measures) {
+                    int bit = measure(h, fm.first, rng.f64());
+                    if (nz) bit = readout(*nz, bit, rng);
+                    cl = (cl & ~(1ull << fm.second)) | ((uint64_t)bit << fm.second);
+                }
+                ++raw[cl];
+            }
+        }
+    } catch (const RunError &e) {
+        return e.code;
+    }
+    out.assign(raw.begin(), raw.end());
+    std::sort(out.begin(), out.end());
+    return ZB_OK;
+}
+
+}  // namespace zeno
+
 extern "C" {
 
 zr_compiled *zr_compile(const zr_instr *prog, int64_t n_instr, uint32_t n_qubits, const uint32_t *creg_sizes,
@@ -206,61 +272,17 @@ int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi) {
 
 int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t seed, const zr_noise *noise, int mode,
                uint64_t *keys, uint64_t *counts, int64_t cap, double *sim_seconds) {
-    const Compiled &c = cc->c;
     if (!valid(noise)) {
         set_error("invalid noise model: every probability must be in [0, 1]");
         return ZB_ERR_ARG;
     }
-    const zr_noise *nz = trivial(noise) ? nullptr : noise;
-    std::unordered_map<uint64_t, uint64_t> raw;
+    std::vector<std::pair<uint64_t, uint64_t>> v;
     const auto t0 = std::chrono::steady_clock::now();
-    try {
-        zb_backend *hp = nullptr;
-        ok(zb_create(c.n_qubits, precision, &hp));
-        BackendPtr h(hp);
-        ok(zb_set_mode(h.get(), mode));
-        if (!nz && !c.dynamic) {  // run_sampled (exec.rs:407-452)
-            for (const Op &o : c.ops) apply_op(h.get(), o);
-            ok(zb_finish(h.get()));
-            if (!c.final_measures.empty() && shots > 0) {
-                std::vector<uint64_t> smp(shots);
-                ok(zb_sample(h.get(), shots, splitmix64(seed), smp.data()));
-                std::unordered_map<uint64_t, uint64_t> basis;
-                for (uint64_t s : smp) ++basis[s];
-                for (const auto &kv : basis) {
-                    uint64_t cl = 0;
-                    for (const auto &fm : c.final_measures) {
-                        const uint64_t bit = (kv.first >> fm.first) & 1;
-                        cl = (cl & ~(1ull << fm.second)) | (bit << fm.second);
-                    }
-                    raw[cl] += kv.second;
-                }
-            }
-        } else if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && zr_trajectories_supported(cc)) {
-            // run_per_shot (exec.rs:459-549) as one batched trajectory launch
-            std::vector<uint64_t> words(shots);
-            ok(zr_run_trajectories(cc, precision, shots, seed, nz, words.data(), nullptr));
-            for (uint64_t w : words) ++raw[w];
-        } else if (c.dynamic || !c.final_measures.empty()) {  // run_per_shot (exec.rs:459-549)
-            for (uint64_t shot = 0; shot < shots; ++shot) {
-                ok(zb_reset_all(h.get()));
-                Rng rng(splitmix64(seed ^ shot));
-                uint64_t cl = 0;
-                for (const Op &o : c.ops) exec_shot_op(h.get(), o, nz, rng, cl);
-                for (const auto &fm : c.final_measures) {
-                    int bit = measure(h.get(), fm.first, rng.f64());
-                    if (nz) bit = readout(*nz, bit, rng);
-                    cl = (cl & ~(1ull << fm.second)) | ((uint64_t)bit << fm.second);
-                }
-                ++raw[cl];
-            }
-        }
-    } catch (const RunError &e) {
-        return e.code;
-    }
+    zb_backend *hp = nullptr;
+    if (const int rc = zb_create(cc->c.n_qubits, precision, &hp); rc < 0) return rc;
+    BackendPtr h(hp);
+    if (const int rc = run_compiled_on(h.get(), cc, shots, seed, noise, mode, true, v); rc < 0) return rc;
     if (sim_seconds) *sim_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    std::vector<std::pair<uint64_t, uint64_t>> v(raw.begin(), raw.end());
-    std::sort(v.begin(), v.end());
     if ((int64_t)v.size() > cap) {
         set_error("key buffer too small");
         return ZB_ERR_ARG;

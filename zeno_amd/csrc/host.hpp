@@ -91,6 +91,14 @@ struct zr_compiled {
 
 namespace zeno {
 
+// exec::run_compiled (src/exec.rs:407-549) on an existing backend whose
+// qubit count matches: resets it to |0...0> (unless the caller says it
+// already is), runs the circuit (sampled,
+// batched trajectories, or per shot) and writes the clbit-word counts,
+// ascending by word.  Returns ZB_OK or a negative ZB_ERR_*.
+int run_compiled_on(zb_backend *h, const zr_compiled *c, uint64_t shots, uint64_t seed, const zr_noise *noise,
+                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out);
+
 // Thread-local last-error plumbing for the C ABI.
 void set_error(const std::string &msg);
 const char *last_error();
