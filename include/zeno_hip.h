@@ -125,11 +125,13 @@ const char *zb_last_error(void);
  * Layer 1 extensions (no counterpart method in the reference trait).
  * ------------------------------------------------------------------- */
 
-/* Raw split state copy to host (T = float or double by precision). */
+/* Raw split state copy to host (T = float or double by precision), in
+ * logical index order. */
 int zb_state_raw(zb_backend *h, void *re, void *im);
-/* Raw split state upload from host. */
+/* Raw split state upload from host (logical index order). */
 int zb_state_set_raw(zb_backend *h, const void *re, const void *im);
-/* Device addresses of the split state (for zero-copy consumers). */
+/* Device addresses of the split state (for zero-copy consumers); the
+ * arrays are in the physical layout of zb_layout. */
 int zb_state_device_ptrs(zb_backend *h, void **re, void **im);
 /* Number of qubits and precision of a handle. */
 uint32_t zb_n_qubits(const zb_backend *h);
@@ -158,9 +160,18 @@ int zb_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
  * (8 B read + 8 B written per element) that scales hardware byte
  * counters.  Allocates and frees its own buffers. */
 int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms);
+/* The backend's qubit layout: phys[q] = the bit of the state index that
+ * holds circuit qubit q (n entries).  Every zb_* call takes and returns
+ * circuit (logical) order; only zb_state_device_ptrs exposes the physical
+ * arrays. */
+int zb_layout(const zb_backend *h, uint32_t *phys);
 /* Tuning knobs of the tiled kernel (flushes first): "prefetch" (0/1:
  * issue the next tile's HBM loads into registers while the current tile
- * computes) and "win_grid" (workgroups per window launch). */
+ * computes), "win_grid" (workgroups per window launch), "slots" (3 or 4
+ * register qubits per stage, 0 = by precision) and "page_bits" (1..12:
+ * most window qubits that select a 2 MiB page of the state, default 6)
+ * and "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
+ * resets the state to |0...0>).  None of them changes results. */
 int zb_tune(zb_backend *h, const char *key, int64_t value);
 /* Launch profiling.  zb_profile_begin flushes, synchronises and starts
  * recording a HIP event pair around every state-sweep kernel launch on the

@@ -126,6 +126,37 @@ def test_tiled_multi_tile_workgroups_bit_exact(orc, prec, tune):
                     _assert_same(g, o, f"n={n} fusion={fusion} tune={tune}")
 
 
+@pytest.mark.parametrize("layout", [1, 5, 11])
+@pytest.mark.parametrize("mode", MODES)
+def test_qubit_layouts_bit_exact(orc, mode, layout):
+    """Any qubit-to-bit layout (the spread one, seeded shuffles) gives the
+    oracle's state, norm, probabilities, chunk sums, samples and collapse."""
+    for n in (12, 14, 17, 20):
+        for prec in (64, 32):
+            prog = orc.gen_matching(n, 8, 40 + n, 3)
+            for fusion in (0, 2, 4):
+                _say(f"layout={layout} mode={mode} n={n} prec={prec} fusion={fusion}")
+                o = orc.State(n, prec)
+                o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+                with z.HipBackend(n, prec, mode) as g:
+                    g.tune("layout", layout)
+                    assert sorted(g.layout()) == list(range(n))
+                    g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                    _assert_same(g, o, f"layout={layout} n={n} fusion={fusion}")
+                    assert g.norm_sqr() == o.norm_sqr()
+                    for q in (0, n // 2, n - 1):
+                        assert g.prob_one(q) == o.prob_one(q), q
+                    assert np.array_equal(g.chunk_sums().view(np.uint64), o.chunk_sums().view(np.uint64))
+                    assert np.array_equal(g.sample(3000, 9), o.sample(3000, 9))
+                    assert g.measure(n - 2, 0.4) == o.measure(n - 2, 0.4)
+                    _assert_same(g, o, "after collapse")
+                    with z.HipBackend(n, prec, mode) as h:  # identity layout, same circuit
+                        h.tune("layout", 0)
+                        h.apply_compiled(z.compile(prog, n, fusion=fusion))
+                        h.measure(n - 2, 0.4)
+                        assert g.max_abs_diff(h) == 0.0
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_adversarial_circuits_bit_exact(orc, mode):
     cases = [(random_mixed(orc, n, 80, 9000 + n), n) for n in (3, 6, 10, 13, 15)]

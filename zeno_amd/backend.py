@@ -178,6 +178,12 @@ class HipBackend:
         _check(lib().zb_max_abs_diff(self._h, other._h, C.byref(out)), "max_abs_diff")
         return out.value
 
+    def layout(self):
+        """phys[q]: the state-index bit that holds circuit qubit q."""
+        out = np.zeros(max(self.n, 1), np.uint32)
+        lib().zb_layout(self._h, ptr(out))
+        return [int(x) for x in out[: self.n]]
+
     def tune(self, key: str, value: int):
         """Set a tiled-kernel knob ("prefetch", "win_grid")."""
         _check(lib().zb_tune(self._h, key.encode(), int(value)), "tune")

@@ -47,11 +47,21 @@ struct zb_backend {
     size_t elem = 8;
     void *re = nullptr, *im = nullptr;
     bool owns_state = true;  // false when bound to caller-owned device memory
+    // qubit layout: circuit qubit q is bit phys[q] of the state index
+    uint8_t phys[64];
+    bool ident = true;
+    int layout_kind = -1;  // -1: by size and precision; 0 identity; 1 spread; >= 2 seeded shuffle
+    uint64_t *lut_dev = nullptr;
+    zb_backend() {
+        for (int i = 0; i < 64; ++i) phys[i] = (uint8_t)i;
+    }
+    dev::Layout layout() const { return dev::Layout{ident ? nullptr : lut_dev}; }
     hipStream_t stream = nullptr;
     int mode = 1;
     bool prefetch = true;  // tiled kernel variant: register prefetch of the next tile
     int slots = 0;          // tiled stage width: 3 (512-lane workgroups) or 4 (256 lanes); 0 = by precision
     unsigned win_grid = 2048;
+    uint32_t page_bits = 6;  // most tile qubits at or above the page bit in one window
     std::string name;
     // parameter arena
     char *arena_dev = nullptr, *arena_host = nullptr;
@@ -161,6 +171,58 @@ void launch_zero(zb_backend *h) {
     check(hipGetLastError(), "k_zero");
 }
 
+// Page-selecting bit: bits at or above it pick a different 2 MiB page of a
+// state array (2^18 doubles, 2^19 floats).
+uint32_t page_bit(const zb_backend *h) { return h->prec == 64 ? 18 : 19; }
+
+// Qubit layout.  Windows are runs of consecutive ops, and circuits touch
+// qubits in runs of consecutive indices, so with the identity layout a
+// window on the top qubits gathers a tile from 2^8 pages per array and the
+// per-CU translation cache (about 16 entries) misses on almost every
+// access.  The spread layout keeps the lowest `low` qubits in place (they
+// carry the coalesced 128 B rows) and deals the rest alternately onto the
+// bits below the page bit and the page-selecting bits, in proportion and
+// each in ascending order, so any run of qubits holds few page bits.
+// Results do not depend on the layout: every op acts on the same
+// amplitudes with the same arithmetic, and every reduction or copy walks
+// logical indices.  Set only while the state is |0...0> (index 0 is fixed).
+void set_layout(zb_backend *h, int kind) {
+    const uint32_t n = h->n, low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, n), pb = page_bit(h);
+    if (kind < 0) kind = n >= pb + 4 ? 1 : 0;
+    for (int i = 0; i < 64; ++i) h->phys[i] = (uint8_t)i;
+    if (kind == 1 && n > pb) {
+        const uint32_t m = n - low, np = n - pb;  // qubits dealt, page bits among them
+        uint32_t next_n = low, next_p = pb, taken_p = 0;
+        for (uint32_t i = 0; i < m; ++i) {
+            const uint32_t want_p = (uint32_t)(((uint64_t)(i + 1) * np) / m);
+            if (taken_p < want_p) h->phys[low + i] = (uint8_t)next_p++, ++taken_p;
+            else h->phys[low + i] = (uint8_t)next_n++;
+        }
+    } else if (kind >= 2 && n > low) {  // seeded shuffle of the bits above `low` (tests)
+        Rng rng((uint64_t)kind);
+        for (uint32_t i = n - 1; i > low; --i) {
+            const uint32_t j = low + (uint32_t)rng.range(0, i - low + 1);
+            std::swap(h->phys[i], h->phys[j]);
+        }
+    }
+    h->layout_kind = kind;
+    h->ident = true;
+    for (uint32_t q = 0; q < n; ++q) h->ident = h->ident && h->phys[q] == q;
+    if (h->ident) return;
+    std::vector<uint64_t> lut(5 * 256, 0);
+    for (int c = 0; c < 5; ++c)
+        for (int v = 0; v < 256; ++v) {
+            uint64_t o = 0;
+            for (int b = 0; b < 8; ++b) {
+                const uint32_t q = (uint32_t)(8 * c + b);
+                if (((v >> b) & 1) && q < n) o |= 1ull << h->phys[q];
+            }
+            lut[(size_t)c * 256 + v] = o;
+        }
+    if (!h->lut_dev) check(hipMalloc((void **)&h->lut_dev, lut.size() * 8), "hipMalloc layout");
+    check(hipMemcpy(h->lut_dev, lut.data(), lut.size() * 8, hipMemcpyHostToDevice), "layout upload");
+}
+
 template <typename T>
 void to_T(const double *src, size_t n, std::vector<T> &re, std::vector<T> &im) {
     re.resize(n);
@@ -169,6 +231,15 @@ void to_T(const double *src, size_t n, std::vector<T> &re, std::vector<T> &im) {
         re[i] = (T)src[2 * i];  // T::fr: round to nearest (src/state.rs:41-50)
         im[i] = (T)src[2 * i + 1];
     }
+}
+
+// State bits of an op's qubits in the op's own order, plus sorted.
+dev::Qubits make_qubits(const uint32_t *qs, uint32_t k) {
+    dev::Qubits q{};
+    q.k = (int)k;
+    for (uint32_t b = 0; b < k; ++b) q.q[b] = q.s[b] = qs[b];
+    std::sort(q.s, q.s + k);
+    return q;
 }
 
 template <typename T, int K>
@@ -201,9 +272,7 @@ void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     std::copy(mr.begin(), mr.end(), packed.begin());
     std::copy(mi.begin(), mi.end(), packed.begin() + dim * dim);
     const T *d = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
-    dev::Qubits q{};
-    q.k = (int)k;
-    for (uint32_t b = 0; b < k; ++b) q.q[b] = qs[b];
+    const dev::Qubits q = make_qubits(qs, k);
     switch (k) {
         case 2: launch_kq<T, 2>(h, d, q); break;
         case 3: launch_kq<T, 3>(h, d, q); break;
@@ -231,9 +300,7 @@ void exact_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     std::copy(dr.begin(), dr.end(), packed.begin());
     std::copy(di.begin(), di.end(), packed.begin() + dim);
     const T *tab = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
-    dev::Qubits q{};
-    q.k = (int)k;
-    for (uint32_t b = 0; b < k; ++b) q.q[b] = qs[b];
+    const dev::Qubits q = make_qubits(qs, k);
     h->timed(ZB_K_DIAGK, h->sweep_bytes(), [&] {
         hipLaunchKernelGGL(dev::k_diagk<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
                            h->I<T>(), tab, q, h->len);
@@ -266,6 +333,14 @@ uint64_t low_mask(const zb_backend *h) {
     return (1ull << low) - 1;
 }
 
+// A tile touches 2^(its page-selecting bits) pages per array; a window
+// keeps at most `page_bits` of them (the layout above makes this rarely
+// bind).
+uint32_t page_bits_of(const zb_backend *h, uint64_t mask) {
+    const uint32_t pb = page_bit(h);
+    return pb >= 64 ? 0 : (uint32_t)__builtin_popcountll(mask >> pb);
+}
+
 // Bytes of the descriptor blob for a window of `nops` ops and `data`
 // bytes of matrices (stages <= ops).
 size_t desc_bytes(size_t nops, size_t data) {
@@ -295,6 +370,7 @@ void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint6
     const bool table = kind == dev::kWDiag;
     const size_t add = table ? 0 : 2 * n_complex * sizeof(T);
     if (!h->win_ops.empty() && ((uint64_t)__builtin_popcountll(h->win_mask | need) > m ||
+                                page_bits_of(h, h->win_mask | need) > h->page_bits ||
                                 desc_bytes(h->win_ops.size() + 1, h->win_data.size() + add) > dev::kDescBytes))
         flush(h);
     if (h->win_ops.empty()) h->win_mask = low_mask(h);
@@ -461,25 +537,39 @@ void flush(zb_backend *h) {
     else flush_T<float>(h);
 }
 
+// The apply_* entry points take circuit qubits and hand state bits to the
+// kernels.  A dense op whose qubits land on state bits in a different order
+// keeps its matrix as written (the per-op kernel gathers in op order); only
+// windows need them ascending.
 template <typename T>
 void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
-    if (!tiled(h) || k > (uint32_t)nslots(h)) {
+    uint32_t pq[dev::kMaxDiagK];
+    bool ascending = true;
+    for (uint32_t b = 0; b < k; ++b) {
+        pq[b] = h->phys[qs[b]];
+        ascending = ascending && (b == 0 || pq[b] > pq[b - 1]);
+    }
+    if (!tiled(h) || k > (uint32_t)nslots(h) || !ascending) {
         flush(h);
-        return exact_unitary_T<T>(h, qs, k, mat);
+        return exact_unitary_T<T>(h, pq, k, mat);
     }
     uint64_t need = 0;
-    for (uint32_t b = 0; b < k; ++b) need |= 1ull << qs[b];
-    enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, qs, k, need, mat, (size_t)1 << (2 * k));
+    for (uint32_t b = 0; b < k; ++b) need |= 1ull << pq[b];
+    enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, pq, k, need, mat, (size_t)1 << (2 * k));
 }
 
 template <typename T>
 void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
-    if (!tiled(h)) return exact_diag_T<T>(h, qs, k, d);
-    enqueue<T>(h, dev::kWDiag, qs, k, 0, d, (size_t)1 << k);
+    uint32_t pq[dev::kMaxDiagK];
+    for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
+    if (!tiled(h)) return exact_diag_T<T>(h, pq, k, d);
+    enqueue<T>(h, dev::kWDiag, pq, k, 0, d, (size_t)1 << k);
 }
 
 template <typename T>
 void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
+    c = h->phys[c];
+    t = h->phys[t];
     if (!tiled(h)) return exact_cx_T<T>(h, c, t);
     const uint32_t qs[2] = {t, c};  // bits[0] = target, bits[1] = control
     enqueue<T>(h, dev::kWCx, qs, 2, 1ull << t, nullptr, 0);
@@ -494,7 +584,7 @@ double reduce_T(zb_backend *h, int mode, uint32_t q) {
     h->ensure_red(2 * nb + 2);
     double *a = h->red_dev, *b = h->red_dev + nb + 1;
     hipLaunchKernelGGL(dev::k_reduce_state<T>, dim3((unsigned)nb), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), n, mode, q, a);
+                       h->I<T>(), h->layout(), n, mode, q, a);
     check(hipGetLastError(), "k_reduce_state");
     while (nb > 1) {
         const uint64_t nn = (nb + dev::kReduceSpan - 1) / dev::kReduceSpan;
@@ -517,7 +607,7 @@ int measure_T(zb_backend *h, uint32_t q, double u) {
     const T scale = (T)(1.0 / std::sqrt(p));
     const uint64_t half = h->len >> 1;
     hipLaunchKernelGGL(dev::k_collapse<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), q, half, outcome, scale);
+                       h->I<T>(), (uint32_t)h->phys[q], half, outcome, scale);
     check(hipGetLastError(), "k_collapse");
     return outcome;
 }
@@ -525,7 +615,7 @@ int measure_T(zb_backend *h, uint32_t q, double u) {
 template <typename T>
 void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
     hipLaunchKernelGGL(dev::k_chunk_sums<T>, dim3((unsigned)nch), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), h->len, chunk, dsums);
+                       h->I<T>(), h->layout(), h->len, chunk, dsums);
     check(hipGetLastError(), "k_chunk_sums");
 }
 
@@ -574,11 +664,61 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
         check(hipMemcpyAsync(dtouched, touched.data(), touched.size() * 4, hipMemcpyHostToDevice, h->stream),
               "touched");
         hipLaunchKernelGGL(dev::k_rescan<T>, dim3((unsigned)touched.size()), dim3(dev::kBlock), 0, h->stream,
-                           h->R<T>(), h->I<T>(), h->len, chunk, dtouched, dcum, dstarts, dus, dout);
+                           h->R<T>(), h->I<T>(), h->layout(), h->len, chunk, dtouched, dcum, dstarts, dus, dout);
         check(hipGetLastError(), "k_rescan");
         check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
     }
     h->sync();
+}
+
+// Host copies of the state in logical index order: straight copies under
+// the identity layout, else chunk by chunk through a device gather/scatter.
+constexpr uint64_t kCopyChunk = 1ull << 22;
+
+template <typename T>
+void read_state_T(zb_backend *h, T *re, T *im) {
+    flush(h);
+    if (h->ident) {
+        check(hipMemcpyAsync(re, h->re, h->len * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy re");
+        check(hipMemcpyAsync(im, h->im, h->len * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy im");
+        h->sync();
+        return;
+    }
+    const uint64_t chunk = std::min<uint64_t>(h->len, kCopyChunk);
+    T *buf = reinterpret_cast<T *>(h->scratch_get(2 * chunk * sizeof(T)));
+    for (uint64_t lo = 0; lo < h->len; lo += chunk) {
+        const uint64_t cnt = std::min<uint64_t>(chunk, h->len - lo);
+        hipLaunchKernelGGL(dev::k_gather<T>, dim3(grid_for(cnt)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), h->layout(), lo, cnt, buf, buf + chunk);
+        check(hipGetLastError(), "k_gather");
+        check(hipMemcpyAsync(re + lo, buf, cnt * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy re");
+        check(hipMemcpyAsync(im + lo, buf + chunk, cnt * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy im");
+        h->sync();
+    }
+}
+
+template <typename T>
+void write_state_T(zb_backend *h, const T *re, const T *im) {
+    h->win_ops.clear();  // the state is being overwritten
+    h->win_data.clear();
+    h->win_tables.clear();
+    if (h->ident) {
+        check(hipMemcpyAsync(h->re, re, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy re");
+        check(hipMemcpyAsync(h->im, im, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy im");
+        h->sync();
+        return;
+    }
+    const uint64_t chunk = std::min<uint64_t>(h->len, kCopyChunk);
+    T *buf = reinterpret_cast<T *>(h->scratch_get(2 * chunk * sizeof(T)));
+    for (uint64_t lo = 0; lo < h->len; lo += chunk) {
+        const uint64_t cnt = std::min<uint64_t>(chunk, h->len - lo);
+        check(hipMemcpyAsync(buf, re + lo, cnt * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy re");
+        check(hipMemcpyAsync(buf + chunk, im + lo, cnt * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy im");
+        hipLaunchKernelGGL(dev::k_scatter<T>, dim3(grid_for(cnt)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), h->layout(), lo, cnt, buf, buf + chunk);
+        check(hipGetLastError(), "k_scatter");
+        h->sync();
+    }
 }
 
 template <typename F>
@@ -667,6 +807,7 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
         create_resources(h);
         check(hipMalloc(&h->re, h->len * h->elem), "hipMalloc re");
         check(hipMalloc(&h->im, h->len * h->elem), "hipMalloc im");
+        set_layout(h, -1);
         zero_state(h);
     });
     if (rc != ZB_OK) {
@@ -691,6 +832,7 @@ int zb_create_bound(uint32_t n_qubits, int precision, void *re, void *im, zb_bac
     h->im = im;
     const int rc = guarded([&] {
         create_resources(h);
+        set_layout(h, -1);
         zero_state(h);
     });
     if (rc != ZB_OK) {
@@ -720,6 +862,7 @@ int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re, void *i
         h->re = re;
         h->im = im;
         set_shape(h, n_qubits, precision);
+        set_layout(h, -1);
         zero_state(h);
     });
 }
@@ -734,6 +877,7 @@ void zb_destroy(zb_backend *h) {
     if (h->red_dev) (void)hipFree(h->red_dev);
     if (h->red_host) (void)hipHostFree(h->red_host);
     if (h->scratch) (void)hipFree(h->scratch);
+    if (h->lut_dev) (void)hipFree(h->lut_dev);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
@@ -841,22 +985,21 @@ int zb_chunk_sums(zb_backend *h, double *out) {
 
 int zb_state_raw(zb_backend *h, void *re, void *im) {
     return guarded([&] {
-        flush(h);
-        check(hipMemcpyAsync(re, h->re, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy re");
-        check(hipMemcpyAsync(im, h->im, h->len * h->elem, hipMemcpyDeviceToHost, h->stream), "copy im");
-        h->sync();
+        if (h->prec == 64) read_state_T<double>(h, (double *)re, (double *)im);
+        else read_state_T<float>(h, (float *)re, (float *)im);
     });
 }
 
 int zb_state_set_raw(zb_backend *h, const void *re, const void *im) {
     return guarded([&] {
-        h->win_ops.clear();  // the state is being overwritten
-        h->win_data.clear();
-        h->win_tables.clear();
-        check(hipMemcpyAsync(h->re, re, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy re");
-        check(hipMemcpyAsync(h->im, im, h->len * h->elem, hipMemcpyHostToDevice, h->stream), "copy im");
-        h->sync();
+        if (h->prec == 64) write_state_T<double>(h, (const double *)re, (const double *)im);
+        else write_state_T<float>(h, (const float *)re, (const float *)im);
     });
+}
+
+int zb_layout(const zb_backend *h, uint32_t *phys) {
+    for (uint32_t q = 0; q < h->n; ++q) phys[q] = h->phys[q];
+    return ZB_OK;
 }
 
 int zb_state_device_ptrs(zb_backend *h, void **re, void **im) {
@@ -872,19 +1015,14 @@ int zb_state_device_ptrs(zb_backend *h, void **re, void **im) {
 
 int zb_statevector(zb_backend *h, double *out) {
     return guarded([&] {
-        flush(h);
         const uint64_t n = h->len;
         if (h->prec == 64) {
             std::vector<double> re(n), im(n);
-            check(hipMemcpyAsync(re.data(), h->re, n * 8, hipMemcpyDeviceToHost, h->stream), "copy re");
-            check(hipMemcpyAsync(im.data(), h->im, n * 8, hipMemcpyDeviceToHost, h->stream), "copy im");
-            h->sync();
+            read_state_T<double>(h, re.data(), im.data());
             for (uint64_t i = 0; i < n; ++i) out[2 * i] = re[i], out[2 * i + 1] = im[i];
         } else {
             std::vector<float> re(n), im(n);
-            check(hipMemcpyAsync(re.data(), h->re, n * 4, hipMemcpyDeviceToHost, h->stream), "copy re");
-            check(hipMemcpyAsync(im.data(), h->im, n * 4, hipMemcpyDeviceToHost, h->stream), "copy im");
-            h->sync();
+            read_state_T<float>(h, re.data(), im.data());
             for (uint64_t i = 0; i < n; ++i) out[2 * i] = (double)re[i], out[2 * i + 1] = (double)im[i];
         }
     });
@@ -963,6 +1101,12 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
         if (k == "prefetch") h->prefetch = value != 0;
         else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
         else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
+        else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
+        else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
+            h->sync();
+            set_layout(h, (int)value);
+            zero_state(h);
+        }
         else throw std::runtime_error("zb_tune: unknown key or bad value: " + k);
     });
 }
@@ -1012,16 +1156,16 @@ int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out) {
         const dim3 g(grid_for(a->len)), blk(dev::kBlock);
         if (a->prec == 64 && b->prec == 64)
             hipLaunchKernelGGL((dev::k_max_abs_diff<double, double>), g, blk, 0, a->stream, a->R<double>(),
-                               a->I<double>(), b->R<double>(), b->I<double>(), a->len, d);
+                               a->I<double>(), a->layout(), b->R<double>(), b->I<double>(), b->layout(), a->len, d);
         else if (a->prec == 64)
             hipLaunchKernelGGL((dev::k_max_abs_diff<double, float>), g, blk, 0, a->stream, a->R<double>(),
-                               a->I<double>(), b->R<float>(), b->I<float>(), a->len, d);
+                               a->I<double>(), a->layout(), b->R<float>(), b->I<float>(), b->layout(), a->len, d);
         else if (b->prec == 64)
             hipLaunchKernelGGL((dev::k_max_abs_diff<float, double>), g, blk, 0, a->stream, a->R<float>(),
-                               a->I<float>(), b->R<double>(), b->I<double>(), a->len, d);
+                               a->I<float>(), a->layout(), b->R<double>(), b->I<double>(), b->layout(), a->len, d);
         else
             hipLaunchKernelGGL((dev::k_max_abs_diff<float, float>), g, blk, 0, a->stream, a->R<float>(),
-                               a->I<float>(), b->R<float>(), b->I<float>(), a->len, d);
+                               a->I<float>(), a->layout(), b->R<float>(), b->I<float>(), b->layout(), a->len, d);
         check(hipGetLastError(), "k_max_abs_diff");
         check(hipMemcpyAsync(a->red_host, d, 8, hipMemcpyDeviceToHost, a->stream), "copy");
         a->sync();

@@ -24,13 +24,32 @@ struct Mat2 {
     T r[4], i[4];
 };
 
+// Qubits of one op: q[b] is the state bit of the op's b-th qubit (the
+// order the matrix or table is written in); s[] holds the same bits sorted
+// ascending, for inserting the zero bits of a group's base index.
 struct Qubits {
     uint32_t q[kMaxDiagK];
+    uint32_t s[kMaxDiagK];
     int k;
 };
 
 __device__ __forceinline__ uint64_t insert_zero(uint64_t x, uint32_t q) {
     return ((x >> q) << (q + 1)) | (x & ((1ull << q) - 1));
+}
+
+// Qubit layout: qubit q of the circuit is bit phys[q] of the state index in
+// HBM.  lut holds 5 tables of 256 entries, one per byte of a logical index,
+// whose OR is the physical index (null = identity).  Reductions, sampling
+// and state copies walk logical indices in the reference's order and read
+// through it; op kernels take physical bits directly.
+struct Layout {
+    const uint64_t *lut;
+};
+
+__device__ __forceinline__ uint64_t phys_index(const Layout &L, uint64_t l) {
+    if (!L.lut) return l;
+    return L.lut[l & 255] | L.lut[256 + ((l >> 8) & 255)] | L.lut[512 + ((l >> 16) & 255)] |
+           L.lut[768 + ((l >> 24) & 255)] | L.lut[1024 + ((l >> 32) & 255)];
 }
 
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
@@ -80,7 +99,7 @@ __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict
     for (uint64_t g = gtid(); g < groups; g += gstride()) {
         uint64_t base = g;
 #pragma unroll
-        for (int b = 0; b < K; ++b) base = insert_zero(base, qs.q[b]);
+        for (int b = 0; b < K; ++b) base = insert_zero(base, qs.s[b]);
         T xr[D], xi[D];
 #pragma unroll
         for (int j = 0; j < D; ++j) {
@@ -199,10 +218,11 @@ __device__ __forceinline__ double block_tree(double a, double *s) {
 }
 
 // mode 0: |amp|^2 over all indices (norm_sqr); mode 1: over the q=1 half
-// indexed by half index (prob_one, src/state.rs:598-626).
+// indexed by half index (prob_one, src/state.rs:598-626).  e and q are
+// logical; the amplitude is read through the layout.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_reduce_state(const T *__restrict__ re, const T *__restrict__ im,
-                                                         uint64_t n_elems, int mode, uint32_t q,
+                                                         Layout L, uint64_t n_elems, int mode, uint32_t q,
                                                          double *__restrict__ out) {
     __shared__ double s[kBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kReduceSpan;
@@ -212,7 +232,7 @@ __global__ __launch_bounds__(kBlock) void k_reduce_state(const T *__restrict__ r
     for (int j = 0; j < kReduceSpan / kBlock; ++j) {
         const uint64_t e = base + (uint64_t)j * kBlock + threadIdx.x;
         if (e < n_elems) {
-            const uint64_t idx = mode ? insert_zero(e, q) + qs : e;
+            const uint64_t idx = phys_index(L, mode ? insert_zero(e, q) + qs : e);
             const double r = (double)re[idx], m = (double)im[idx];
             a = a + (r * r + m * m);
         }
@@ -627,12 +647,14 @@ static __global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ dst
 // so the grid combines block maxima with an integer atomicMax.
 template <typename A, typename B>
 __global__ __launch_bounds__(kBlock) void k_max_abs_diff(const A *__restrict__ ar, const A *__restrict__ ai,
-                                                         const B *__restrict__ br, const B *__restrict__ bi,
-                                                         uint64_t len, unsigned long long *__restrict__ out) {
+                                                         Layout la, const B *__restrict__ br,
+                                                         const B *__restrict__ bi, Layout lb, uint64_t len,
+                                                         unsigned long long *__restrict__ out) {
     __shared__ double s[kBlock];
     double m = 0.0;
     for (uint64_t i = gtid(); i < len; i += gstride()) {
-        const double dr = (double)ar[i] - (double)br[i], di = (double)ai[i] - (double)bi[i];
+        const uint64_t pa = phys_index(la, i), pb = phys_index(lb, i);
+        const double dr = (double)ar[pa] - (double)br[pb], di = (double)ai[pa] - (double)bi[pb];
         m = fmax(m, sqrt(dr * dr + di * di));
     }
     s[threadIdx.x] = m;
@@ -650,7 +672,7 @@ __global__ __launch_bounds__(kBlock) void k_max_abs_diff(const A *__restrict__ a
 // element slab coalesced and square it into LDS; lane 0 then accumulates
 // the slab serially, so the sum order is exactly the reference's.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re, const T *__restrict__ im,
+__global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re, const T *__restrict__ im, Layout L,
                                                        uint64_t len, uint64_t chunk, double *__restrict__ sums) {
     __shared__ double tv[kReduceSpan];
     const uint64_t lo = (uint64_t)blockIdx.x * chunk;
@@ -659,7 +681,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re,
     for (uint64_t slab = lo; slab < hi; slab += kReduceSpan) {
         const int cnt = (int)(hi - slab < (uint64_t)kReduceSpan ? hi - slab : (uint64_t)kReduceSpan);
         for (int j = threadIdx.x; j < cnt; j += kBlock) {
-            const double r = (double)re[slab + j], m = (double)im[slab + j];
+            const uint64_t p = phys_index(L, slab + j);
+            const double r = (double)re[p], m = (double)im[p];
             tv[j] = r * r + m * m;
         }
         __syncthreads();
@@ -675,7 +698,8 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re,
 // below the running sum; leftovers land on the chunk's last index with
 // nonzero probability.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, uint64_t len,
+__global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, Layout L,
+                                                   uint64_t len,
                                                    uint64_t chunk, const uint32_t *__restrict__ touched,
                                                    const double *__restrict__ cum, const uint64_t *__restrict__ starts,
                                                    const double *__restrict__ us, uint64_t *__restrict__ out) {
@@ -696,7 +720,8 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
     for (uint64_t slab = lo; slab < hi; slab += kReduceSpan) {
         const int cnt = (int)(hi - slab < (uint64_t)kReduceSpan ? hi - slab : (uint64_t)kReduceSpan);
         for (int j = threadIdx.x; j < cnt; j += kBlock) {
-            const double r = (double)re[slab + j], m = (double)im[slab + j];
+            const uint64_t p = phys_index(L, slab + j);
+            const double r = (double)re[p], m = (double)im[p];
             tv[j] = r * r + m * m;
         }
         __syncthreads();
@@ -718,7 +743,8 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
     uint64_t best = 0;
     int found = 0;
     for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
-        const double r = (double)re[i], m = (double)im[i];
+        const uint64_t p = phys_index(L, i);
+        const double r = (double)re[p], m = (double)im[p];
         if (r * r + m * m > 0.0) best = i, found = 1;
     }
     slast[threadIdx.x] = found ? best + 1 : 0;
@@ -728,6 +754,30 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
         for (int t = 0; t < kBlock; ++t) m = slast[t] > m ? slast[t] : m;
         const uint64_t last = m ? m - 1 : hi - 1;
         for (uint64_t p = sptr; p < s1; ++p) out[p] = last;
+    }
+}
+
+// State copies in logical order through the layout: out[j] = state at
+// logical index lo + j (gather), or the reverse (scatter).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gather(const T *__restrict__ re, const T *__restrict__ im, Layout L,
+                                                   uint64_t lo, uint64_t count, T *__restrict__ ore,
+                                                   T *__restrict__ oim) {
+    for (uint64_t j = gtid(); j < count; j += gstride()) {
+        const uint64_t p = phys_index(L, lo + j);
+        ore[j] = re[p];
+        oim[j] = im[p];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_scatter(T *__restrict__ re, T *__restrict__ im, Layout L, uint64_t lo,
+                                                    uint64_t count, const T *__restrict__ ire,
+                                                    const T *__restrict__ iim) {
+    for (uint64_t j = gtid(); j < count; j += gstride()) {
+        const uint64_t p = phys_index(L, lo + j);
+        re[p] = ire[j];
+        im[p] = iim[j];
     }
 }
 
