@@ -108,12 +108,14 @@ def test_random_circuits_bit_exact(orc, prec, mode):
 
 
 @pytest.mark.parametrize("tune", [{"win_grid": 3, "prefetch": 1}, {"win_grid": 5, "prefetch": 0},
-                                  {"win_grid": 2, "slots": 4}, {"win_grid": 4, "slots": 3}])
+                                  {"win_grid": 2, "slots": 4}, {"win_grid": 4, "slots": 3},
+                                  {"tile_bits": 13, "win_grid": 3}, {"tile_bits": 13, "slots": 4, "prefetch": 0},
+                                  {"tile_bits": 13}])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_tiled_multi_tile_workgroups_bit_exact(orc, prec, tune):
     """A small window grid makes every workgroup walk many tiles (the
     prefetch and write-back hand-over between consecutive tiles)."""
-    for n in (12, 14, 17):
+    for n in (12, 13, 14, 17):
         for prog in (orc.gen_random(n, 10, 77 + n), orc.gen_matching(n, 10, 5 + n, 3)):
             for fusion in (1, 3):
                 _say(f"multi-tile n={n} fusion={fusion} prec={prec} tune={tune}")

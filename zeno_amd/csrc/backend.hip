@@ -62,6 +62,7 @@ struct zb_backend {
     int slots = 0;          // tiled stage width: 3 (512-lane workgroups) or 4 (256 lanes); 0 = by precision
     unsigned win_grid = 2048;
     uint32_t page_bits = 6;  // most tile qubits at or above the page bit in one window
+    uint32_t tile_bits = 0;  // tile width of the tiled kernel: 12 or 13 (0 = 12)
     std::string name;
     // parameter arena
     char *arena_dev = nullptr, *arena_host = nullptr;
@@ -324,9 +325,10 @@ void exact_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
 // tile) stays within the tile width; otherwise the window is flushed as one
 // k_window pass.  A window holding a single op runs the per-op kernel, and
 // so do dense ops on 5 or 6 qubits and every op of a state under 12 qubits.
-uint32_t tile_bits(const zb_backend *h) { return std::min<uint32_t>(h->n, dev::kTileBits); }
+uint32_t tile_width(const zb_backend *h) { return h->tile_bits ? h->tile_bits : 12; }
+uint32_t tile_bits(const zb_backend *h) { return std::min<uint32_t>(h->n, tile_width(h)); }
 // Tiled windows need n >= 12 (full tiles) and 32-bit lane offsets (n <= 33).
-bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= (uint32_t)dev::kTileBits && h->n <= 33; }
+bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= tile_width(h) && h->n <= 33; }
 
 uint64_t low_mask(const zb_backend *h) {
     const uint32_t low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, h->n);
@@ -350,17 +352,34 @@ size_t desc_bytes(size_t nops, size_t data) {
 
 int nslots(const zb_backend *h) { return h->slots ? h->slots : (h->prec == 64 ? 3 : 4); }
 
-template <typename T, int NS>
-void launch_window(zb_backend *h, bool dense, unsigned grid, const uint32_t *blob, const T *tab, uint64_t tiles) {
-    const dim3 g(grid), b(dev::WinShape<NS>::kLanes);
+template <typename T, int TB, int NS>
+void launch_window_shape(zb_backend *h, bool dense, unsigned grid, const uint32_t *blob, const T *tab,
+                         uint64_t tiles) {
+    const dim3 g(grid), b(dev::WinShape<T, TB, NS>::kLanes);
     if (dense && h->prefetch)
-        hipLaunchKernelGGL((dev::k_window<T, NS, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
+                           tab, tiles);
     else if (dense)
-        hipLaunchKernelGGL((dev::k_window<T, NS, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
+                           tab, tiles);
     else if (h->prefetch)
-        hipLaunchKernelGGL((dev::k_window<T, NS, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
+                           tab, tiles);
     else
-        hipLaunchKernelGGL((dev::k_window<T, NS, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob, tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
+                           tab, tiles);
+}
+
+template <typename T>
+void launch_window(zb_backend *h, int ns, bool dense, unsigned grid, const uint32_t *blob, const T *tab,
+                   uint64_t tiles) {
+    if (tile_width(h) == 13) {
+        if (ns == 3) launch_window_shape<T, 13, 3>(h, dense, grid, blob, tab, tiles);
+        else launch_window_shape<T, 13, 4>(h, dense, grid, blob, tab, tiles);
+    } else {
+        if (ns == 3) launch_window_shape<T, 12, 3>(h, dense, grid, blob, tab, tiles);
+        else launch_window_shape<T, 12, 4>(h, dense, grid, blob, tab, tiles);
+    }
 }
 
 template <typename T>
@@ -522,8 +541,7 @@ void flush_T(zb_backend *h) {
     const uint64_t tiles = h->len >> m;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, h->win_grid);
     h->timed(ZB_K_WINDOW, h->sweep_bytes(), [&] {
-        if (ns == 3) launch_window<T, 3>(h, dense, grid, dblob, dtab, tiles);
-        else launch_window<T, 4>(h, dense, grid, dblob, dtab, tiles);
+        launch_window<T>(h, ns, dense, grid, dblob, dtab, tiles);
     });
     check(hipGetLastError(), "k_window");
     h->win_ops.clear();
@@ -1102,6 +1120,7 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
         else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
         else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
         else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
+        else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
         else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
             h->sync();
             set_layout(h, (int)value);

@@ -264,14 +264,12 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__re
 // stages: a stage fixes NS local "slot" bits; each lane gathers the 2^NS
 // amplitudes that differ only in those bits into registers, applies the
 // stage's ops there with exactly the per-op kernels' arithmetic, and
-// scatters them back.  A workgroup has 2^(12-NS) lanes.  Ops admitted to a
+// scatters them back.  A workgroup has 2^(TB-NS) lanes.  Ops admitted to a
 // window: 1q dense and dense k <= NS whose qubits are slots of one stage;
 // diagonals on any qubits (elementwise: bits outside Q are fixed per tile);
 // CX whose target is a slot (its control may be a slot, another local bit,
 // or fixed per tile).  Window descriptors are uniform (scalar loads).
-constexpr int kTileBits = 12;
-constexpr int kTile = 1 << kTileBits;
-constexpr int kTilePad = kTile + (kTile >> 4);
+constexpr int kMaxTileBits = 13;
 constexpr int kMaxSlots = 4;
 constexpr uint32_t kFixedBit = 0x80000000u;  // table bit outside Q: qubit in the low bits
 constexpr int kDescBytes = 65536;           // window descriptor blob limit
@@ -297,7 +295,7 @@ struct WinStage {
 struct WinHeader {
     uint32_t m, nstages, nops, desc_words;
     uint32_t stage_off, op_off, data_off, nslots;  // word offsets inside the blob
-    uint32_t q[kTileBits];
+    uint32_t q[kMaxTileBits];
 };
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
@@ -417,13 +415,22 @@ __device__ __forceinline__ void reg_dense_dispatch(T *xr, T *xi, uint32_t mask, 
     }
 }
 
-// Workgroup size of the tiled kernel for NS slots, and the occupancy asked
-// of the compiler (workgroups per CU; the f64 tile uses 68 KB of LDS, two
-// per CU).
-template <int NS>
+// Shape of the tiled kernel: a tile of 2^TB amplitudes, NS register
+// slots per lane, hence 2^(TB-NS) lanes; the padded tile lives in LDS and
+// fixes how many workgroups share a CU (160 KiB of LDS): the 12-bit f64
+// tile takes 68 KiB (two per CU), the 13-bit one 136 KiB (one).
+template <typename T, int TB, int NS>
 struct WinShape {
+    static constexpr int kTile = 1 << TB;
+    static constexpr int kTilePad = kTile + (kTile >> 4);
     static constexpr int kLanes = kTile >> NS;
     static constexpr int kRegs = 1 << NS;
+    static constexpr int kLdsBytes = 2 * kTilePad * (int)sizeof(T);
+    static constexpr int kPerCu = (160 * 1024) / kLdsBytes < 1 ? 1 : (160 * 1024) / kLdsBytes;
+    // occupancy asked of the compiler: two workgroups per CU for 12-bit
+    // tiles, one for 13-bit ones, at most 4 waves per SIMD (128 VGPRs)
+    static constexpr int kWant = (TB == 12 ? 2 : 1) * kLanes / 256;
+    static constexpr int kWavesPerEu = kWant < 1 ? 1 : (kWant > 4 ? 4 : kWant);
 };
 
 // kDense: the window holds dense k >= 2 ops; kPrefetch: the next tile's HBM
@@ -431,13 +438,15 @@ struct WinShape {
 // keep the register budget of the common 1q/CX/diagonal windows low).
 // Global addressing: row j of a tile starts at the uniform element offset
 // base | hi(j); a lane adds its offset lo (its low local bits).
-template <typename T, int NS, bool kDense, bool kPrefetch>
-__global__ __launch_bounds__(WinShape<NS>::kLanes, 2) void k_window(T *__restrict__ re, T *__restrict__ im,
-                                                                   const uint32_t *__restrict__ blob,
-                                                                   const T *__restrict__ tables, uint64_t tiles) {
-    constexpr int B = WinShape<NS>::kLanes, R = WinShape<NS>::kRegs;
-    constexpr int LB = kTileBits - NS;  // local bits carried by the lane index
-    __shared__ T sre[kTilePad], sim[kTilePad];
+template <typename T, int TB, int NS, bool kDense, bool kPrefetch>
+__global__ __attribute__((amdgpu_flat_work_group_size(1, WinShape<T, TB, NS>::kLanes)))
+__attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_window(
+    T *__restrict__ re, T *__restrict__ im, const uint32_t *__restrict__ blob, const T *__restrict__ tables,
+    uint64_t tiles) {
+    using Shape = WinShape<T, TB, NS>;
+    constexpr int B = Shape::kLanes, R = Shape::kRegs;
+    constexpr int LB = TB - NS;  // local bits carried by the lane index
+    __shared__ T sre[Shape::kTilePad], sim[Shape::kTilePad];
     const int t = threadIdx.x;
     const WinHeader *hdr = reinterpret_cast<const WinHeader *>(blob);
     const WinStage *stages = reinterpret_cast<const WinStage *>(blob + hdr->stage_off);
@@ -452,13 +461,13 @@ __global__ __launch_bounds__(WinShape<NS>::kLanes, 2) void k_window(T *__restric
     auto tile_base = [&](uint64_t tile) {
         uint64_t b = tile;
 #pragma unroll
-        for (int i = 0; i < kTileBits; ++i) b = insert_zero(b, hdr->q[i]);
+        for (int i = 0; i < TB; ++i) b = insert_zero(b, hdr->q[i]);
         return b;
     };
     auto row = [&](uint64_t base, int j) {
         uint64_t h = base;
 #pragma unroll
-        for (int i = LB; i < kTileBits; ++i)
+        for (int i = LB; i < TB; ++i)
             if ((j >> (i - LB)) & 1) h |= 1ull << hdr->q[i];
         return h;
     };
