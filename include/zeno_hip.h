@@ -169,10 +169,17 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * issue the next tile's HBM loads into registers while the current tile
  * computes), "win_grid" (workgroups per window launch), "slots" (3 or 4
  * register qubits per stage, 0 = by precision) and "page_bits" (1..12:
- * most window qubits that select a 2 MiB page of the state, default 6)
- * and "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
+ * most window qubits that select a 2 MiB page of the state, default 6),
+ * "tile_bits" (12 or 13: tile width of the tiled kernel, 0 = 12) and
+ * "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
  * resets the state to |0...0>).  None of them changes results. */
 int zb_tune(zb_backend *h, const char *key, int64_t value);
+/* Process-wide default for a tuning knob of zb_tune, applied to every
+ * backend of n_qubits at this precision when it is created or re-bound
+ * (value INT64_MIN removes it).  The autotuner installs its choices
+ * here. */
+int zb_tune_default(uint32_t n_qubits, int precision, const char *key,
+                    int64_t value);
 /* Launch profiling.  zb_profile_begin flushes, synchronises and starts
  * recording a HIP event pair around every state-sweep kernel launch on the
  * backend's stream.  zb_profile_end flushes, synchronises, stops, and

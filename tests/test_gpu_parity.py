@@ -424,3 +424,27 @@ def test_serving_admission_limits():
         for i in ids:
             r = srv.wait(i)
             assert r.raw_counts == {}
+
+
+def test_autotune_and_installed_defaults_keep_results(orc, tmp_path):
+    """The autotuner's choices (and any installed knob defaults) leave every
+    result bit-identical: only launch shapes change."""
+    n = 14
+    entry = z.tune.autotune(n, 64, depth=3, reps=1, path=str(tmp_path / "t.json"),
+                            search={"tile_bits": [13], "slots": [4], "prefetch": [0]}, fusions=[1, 2])
+    assert set(entry) >= {"knobs", "fusion", "ms"} and entry["fusion"] in (1, 2)
+    prog = orc.gen_matching(n, 8, 3, 3)
+    o = orc.State(n, 64)
+    o.apply_compiled(orc.Compiled(prog, n, fusion=1))
+    try:
+        for knobs in (entry["knobs"], {"tile_bits": 13, "slots": 4, "layout": 5, "win_grid": 3}):
+            z.tune.install(n, 64, knobs)
+            with z.HipBackend(n, 64) as g:
+                g.apply_compiled(z.compile(prog, n, fusion=1))
+                _assert_same(g, o, f"installed {knobs}")
+            r = z.run_program(orc.program([prog, orc.measure_all(n)]), n,
+                              opts=z.RunOptions(shots=500, seed=3, precision=64))
+            want, _ = orc.run(orc.Compiled(orc.program([prog, orc.measure_all(n)]), n, fusion=1), 500, 3)
+            assert r.raw_counts == want
+    finally:
+        z.tune.uninstall(n, 64)

@@ -105,3 +105,30 @@ def test_teleport_compiles_dynamic():
             z.ir.measure(2, 2)]
     c = z.compile(z.ir.program(rows), 3, cregs=[1, 1, 1])
     assert c.dynamic and c.op(7)["is_if"] and c.op(7)["cond"] == [1, 1, 1]
+
+
+def test_tune_cache_roundtrip(tmp_path):
+    path = str(tmp_path / "tune.json")
+    assert z.tune.load_cache(path) == {}
+    entry = {"knobs": {"tile_bits": 13, "slots": 4}, "fusion": 3, "ms": {}}
+    z.tune.save_cache({"20:64": entry}, path)
+    assert z.tune.load_cache(path)["20:64"] == entry
+    assert z.tune.tuned_fusion(20, 64, path) == 3
+    assert z.tune.tuned_fusion(21, 64, path) is None
+    assert z.tune.load(path) == 1  # installs into the native defaults table (no device needed)
+    z.tune.uninstall(20, 64)
+    lib = z._lib.lib()
+    assert lib.zb_tune_default(20, 64, b"no_such_knob", 1) < 0
+    assert lib.zb_tune_default(20, 16, b"slots", 3) < 0
+
+
+def test_fusion_auto_uses_cache(tmp_path, monkeypatch):
+    path = str(tmp_path / "tune.json")
+    z.tune.save_cache({"9:64": {"knobs": {}, "fusion": 4}}, path)
+    monkeypatch.setenv("ZENO_TUNE_CACHE", path)
+    from zeno_amd.exec import compile_options
+
+    assert compile_options(z.RunOptions(fusion_max="auto", precision=64), 9) == 4
+    assert compile_options(z.RunOptions(fusion_max="auto", precision=64), 10) == 1
+    assert compile_options(z.RunOptions(fusion_max=2), 9) == 2
+    assert compile_options(z.RunOptions(fusion_max="auto", noise=z.NoiseModel(bit_flip=0.1)), 9) == 0

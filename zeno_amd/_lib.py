@@ -45,6 +45,7 @@ SIGNATURES = {
     "zb_max_abs_diff": (I32, [P, P, P]),
     "zb_profile_begin": (I32, [P]),
     "zb_tune": (I32, [P, C.c_char_p, I64]),
+    "zb_tune_default": (I32, [U32, I32, C.c_char_p, I64]),
     "zb_profile_end": (I64, [P, P, P, P, I64]),
     "zr_compile": (P, [P, I64, U32, P, U32, I32, I32]),
     "zr_compiled_free": (None, [P]),

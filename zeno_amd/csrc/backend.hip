@@ -9,6 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 #include <limits>
@@ -769,6 +772,32 @@ const char *zb_last_error(void) { return last_error(); }
 
 namespace {
 
+// Process-wide knob defaults per (qubits, precision), applied to every
+// backend of that shape when it is created or re-bound (the autotuner
+// installs its choices here).
+std::mutex g_tune_mu;
+std::map<std::tuple<uint32_t, int, std::string>, int64_t> g_tune_defaults;
+
+int apply_knob(zb_backend *h, const std::string &k, int64_t value);
+
+void apply_defaults(zb_backend *h) {
+    std::vector<std::pair<std::string, int64_t>> kv;
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mu);
+        for (const auto &e : g_tune_defaults)
+            if (std::get<0>(e.first) == h->n && std::get<1>(e.first) == h->prec)
+                kv.emplace_back(std::get<2>(e.first), e.second);
+    }
+    for (const auto &p : kv)
+        if (p.first != "layout") apply_knob(h, p.first, p.second);
+}
+
+int64_t default_layout(const zb_backend *h) {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_defaults.find({h->n, h->prec, "layout"});
+    return it == g_tune_defaults.end() ? -1 : it->second;
+}
+
 int check_shape(uint32_t n_qubits, int precision) {
     if (precision != 32 && precision != 64) {
         set_error("precision must be 32 or 64");
@@ -807,6 +836,22 @@ void zero_state(zb_backend *h) {
     h->sync();
 }
 
+// One tuning knob (none changes results); ZB_ERR_ARG for an unknown key
+// or value.  "layout" resets the state to |0...0>.
+int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
+    if (k == "prefetch") h->prefetch = value != 0;
+    else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
+    else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
+    else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
+    else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
+    else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
+        h->sync();
+        set_layout(h, (int)value);
+        zero_state(h);
+    } else return ZB_ERR_ARG;
+    return ZB_OK;
+}
+
 }  // namespace
 
 int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
@@ -825,7 +870,8 @@ int zb_create(uint32_t n_qubits, int precision, zb_backend **out) {
         create_resources(h);
         check(hipMalloc(&h->re, h->len * h->elem), "hipMalloc re");
         check(hipMalloc(&h->im, h->len * h->elem), "hipMalloc im");
-        set_layout(h, -1);
+        set_layout(h, (int)default_layout(h));
+        apply_defaults(h);
         zero_state(h);
     });
     if (rc != ZB_OK) {
@@ -850,7 +896,8 @@ int zb_create_bound(uint32_t n_qubits, int precision, void *re, void *im, zb_bac
     h->im = im;
     const int rc = guarded([&] {
         create_resources(h);
-        set_layout(h, -1);
+        set_layout(h, (int)default_layout(h));
+        apply_defaults(h);
         zero_state(h);
     });
     if (rc != ZB_OK) {
@@ -880,7 +927,8 @@ int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re, void *i
         h->re = re;
         h->im = im;
         set_shape(h, n_qubits, precision);
-        set_layout(h, -1);
+        set_layout(h, (int)default_layout(h));
+        apply_defaults(h);
         zero_state(h);
     });
 }
@@ -1116,18 +1164,21 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
     const std::string k(key ? key : "");
     return guarded([&] {
         flush(h);
-        if (k == "prefetch") h->prefetch = value != 0;
-        else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
-        else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
-        else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
-        else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
-        else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
-            h->sync();
-            set_layout(h, (int)value);
-            zero_state(h);
-        }
-        else throw std::runtime_error("zb_tune: unknown key or bad value: " + k);
+        if (apply_knob(h, k, value) != ZB_OK) throw std::runtime_error("zb_tune: unknown key or bad value: " + k);
     });
+}
+
+int zb_tune_default(uint32_t n_qubits, int precision, const char *key, int64_t value) {
+    static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits", "layout"};
+    const std::string k(key ? key : "");
+    if (std::find(std::begin(keys), std::end(keys), k) == std::end(keys) || (precision != 32 && precision != 64)) {
+        set_error("zb_tune_default: unknown key or precision: " + k);
+        return ZB_ERR_ARG;
+    }
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (value == INT64_MIN) g_tune_defaults.erase({n_qubits, precision, k});
+    else g_tune_defaults[{n_qubits, precision, k}] = value;
+    return ZB_OK;
 }
 
 int zb_profile_begin(zb_backend *h) {

@@ -209,7 +209,7 @@ class RunOptions:
     shots: int = 1024
     seed: Optional[int] = None
     precision: Optional[int] = None  # None = auto: f64 if it fits, else f32
-    fusion_max: Optional[int] = None  # None = 1 (the CPU path's default)
+    fusion_max: Optional[object] = None  # None = 1 (the CPU path's default); "auto" = autotuned width
     mem_limit: Optional[int] = None
     want_statevector: bool = False
     noise: Optional[NoiseModel] = None
@@ -325,18 +325,24 @@ def _splitmix64(x: int) -> int:
     return z ^ (z >> 31)
 
 
-def compile_options(opts: RunOptions) -> int:
+def compile_options(opts: RunOptions, n_qubits: Optional[int] = None) -> int:
     """lib.rs:224-240: noisy runs compile with fusion 0; the default dense
-    fusion width is 1 (the reference CPU default)."""
+    fusion width is 1 (the reference CPU default); "auto" takes the width
+    the autotuner measured fastest for this size (1 if none is cached)."""
     if opts.noise is not None and not opts.noise.is_trivial():
         return 0
+    if opts.fusion_max == "auto":
+        from .tune import tuned_fusion
+
+        k = tuned_fusion(n_qubits, opts.precision or 64) if n_qubits is not None else None
+        return 1 if k is None else k
     return 1 if opts.fusion_max is None else opts.fusion_max
 
 
 def run_program(prog, n_qubits: int, cregs=None, opts: Optional[RunOptions] = None) -> RunResult:
     """lib.rs:242-250 run_program."""
     opts = opts or RunOptions()
-    return run_compiled(compile(prog, n_qubits, cregs, fusion=compile_options(opts)), opts)
+    return run_compiled(compile(prog, n_qubits, cregs, fusion=compile_options(opts, n_qubits)), opts)
 
 
 class Simulator:
