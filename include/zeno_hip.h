@@ -310,6 +310,15 @@ int64_t zr_gen_u3_qft(uint32_t n, uint64_t seed, zr_instr *out, int64_t cap);
 int64_t zr_gen_matching(uint32_t n, uint32_t depth, uint64_t seed,
                         uint32_t rzz_period, zr_instr *out, int64_t cap);
 
+/* OpenQASM 2.0 front end (src/qasm.rs:70-76 parse_str; subset and rules
+ * of docs/QASM.md): writes up to `cap` instructions, the qubit count and
+ * up to cregs_cap classical register sizes; returns the instruction count
+ * (call again with a larger cap if it exceeds cap) or < 0 with
+ * "line L:C: message" in zb_last_error(). */
+int64_t zr_parse_qasm(const char *src, zr_instr *out, int64_t cap,
+                      uint32_t *n_qubits, uint32_t *cregs, uint32_t *n_cregs,
+                      uint32_t cregs_cap);
+
 /* gates::build (src/gates.rs:119-188): returns the arity or -1. */
 int zr_gate_build(const char *name, const double *params, int np,
                   double *out, int *is_diag);

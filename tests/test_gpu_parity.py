@@ -448,3 +448,25 @@ def test_autotune_and_installed_defaults_keep_results(orc, tmp_path):
             assert r.raw_counts == want
     finally:
         z.tune.uninstall(n, 64)
+
+
+def test_qasm_programs_run_like_the_oracle(orc):
+    """QASM front end -> compiler -> HIP backend, counts against the oracle
+    on the same parsed rows (static, dynamic and user-gate programs)."""
+    hdr = 'OPENQASM 2.0;\ninclude "qelib1.inc";\n'
+    progs = [
+        hdr + "qreg q[14];\ncreg c[14];\nh q[0];\n" + "".join(f"cx q[{i}],q[{i + 1}];\n" for i in range(13)) +
+        "measure q -> c;\n",
+        hdr + "qreg q[13];\ncreg c[13];\ngate cphase(t) a, b { cp(t) a, b; }\nh q;\n" +
+        "".join(f"cphase(pi/{2 ** (j % 5 + 1)}) q[{j}], q[{(3 * j + 1) % 13}];\n" for j in range(12)) +
+        "u3(0.3, 0.2, 0.1) q;\nmeasure q -> c;\n",
+        hdr + "qreg q[3];\ncreg a[1];\ncreg b[1];\ncreg r[1];\nx q[0];\nh q[1];\ncx q[1],q[2];\ncx q[0],q[1];\n"
+        "h q[0];\nmeasure q[0] -> a[0];\nmeasure q[1] -> b[0];\nif (b == 1) x q[2];\nif (a == 1) z q[2];\n"
+        "measure q[2] -> r[0];\n",
+    ]
+    for i, src in enumerate(progs):
+        _say(f"qasm program {i}")
+        p = z.qasm.parse(src)
+        want, _ = orc.run(orc.Compiled(p.rows, p.n_qubits, cregs=p.cregs, fusion=1), 3000, 5 + i)
+        got = z.qasm.run_qasm(src, z.RunOptions(shots=3000, seed=5 + i, precision=64))
+        assert got.raw_counts == want

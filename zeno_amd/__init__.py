@@ -37,11 +37,11 @@ from .exec import (
 )
 from . import ir
 from .serve import Served, Server
-from . import tune
+from . import qasm, tune
 
 __all__ = [
     "Circuit", "Compiled", "calibration_copy", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
     "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
-    "format_key", "ir", "tune", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
+    "format_key", "ir", "qasm", "tune", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
     "Served", "Server", "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",
 ]

@@ -66,6 +66,7 @@ SIGNATURES = {
     "zs_wait": (I64, [P, I64, P, P, I64, P]),
     "zs_status": (None, [P, P]),
     "zs_destroy": (None, [P]),
+    "zr_parse_qasm": (I64, [C.c_char_p, P, I64, P, P, P, U32]),
     "zr_trajectories_supported": (I32, [P]),
     "zr_run_trajectories": (I32, [P, I32, U64, U64, P, P, P]),
     "zr_gen_random": (I64, [U32, U32, U64, P, I64]),
