@@ -458,7 +458,7 @@ def test_qasm_programs_run_like_the_oracle(orc):
         hdr + "qreg q[14];\ncreg c[14];\nh q[0];\n" + "".join(f"cx q[{i}],q[{i + 1}];\n" for i in range(13)) +
         "measure q -> c;\n",
         hdr + "qreg q[13];\ncreg c[13];\ngate cphase(t) a, b { cp(t) a, b; }\nh q;\n" +
-        "".join(f"cphase(pi/{2 ** (j % 5 + 1)}) q[{j}], q[{(3 * j + 1) % 13}];\n" for j in range(12)) +
+        "".join(f"cphase(pi/{2 ** (j % 5 + 1)}) q[{j}], q[{(j + 5) % 13}];\n" for j in range(12)) +
         "u3(0.3, 0.2, 0.1) q;\nmeasure q -> c;\n",
         hdr + "qreg q[3];\ncreg a[1];\ncreg b[1];\ncreg r[1];\nx q[0];\nh q[1];\ncx q[1],q[2];\ncx q[0],q[1];\n"
         "h q[0];\nmeasure q[0] -> a[0];\nmeasure q[1] -> b[0];\nif (b == 1) x q[2];\nif (a == 1) z q[2];\n"
