@@ -170,7 +170,10 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * computes), "win_grid" (workgroups per window launch), "slots" (3 or 4
  * register qubits per stage, 0 = by precision) and "page_bits" (1..12:
  * most window qubits that select a 2 MiB page of the state, default 6),
- * "tile_bits" (12 or 13: tile width of the tiled kernel, 0 = 12) and
+ * "tile_bits" (12 or 13: tile width of the tiled kernel, 0 = 12),
+ * "defer_cx" (0/1, default 1: a CX whose target is outside the open window
+ * waits, bit-exactly, for a window that holds it; a CX is a permutation and
+ * commutes with every op on other qubits) and
  * "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
  * resets the state to |0...0>).  None of them changes results. */
 int zb_tune(zb_backend *h, const char *key, int64_t value);

@@ -110,7 +110,7 @@ def test_random_circuits_bit_exact(orc, prec, mode):
 @pytest.mark.parametrize("tune", [{"win_grid": 3, "prefetch": 1}, {"win_grid": 5, "prefetch": 0},
                                   {"win_grid": 2, "slots": 4}, {"win_grid": 4, "slots": 3},
                                   {"tile_bits": 13, "win_grid": 3}, {"tile_bits": 13, "slots": 4, "prefetch": 0},
-                                  {"tile_bits": 13}])
+                                  {"tile_bits": 13}, {"defer_cx": 0, "win_grid": 3}])
 @pytest.mark.parametrize("prec", [64, 32])
 def test_tiled_multi_tile_workgroups_bit_exact(orc, prec, tune):
     """A small window grid makes every workgroup walk many tiles (the

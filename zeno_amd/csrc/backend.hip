@@ -85,6 +85,11 @@ struct zb_backend {
     };
     uint64_t win_mask = 0;
     std::vector<POp> win_ops;
+    // CX ops held back (physical control, target; pairwise disjoint): a CX is a
+    // pure permutation, so it commutes bit-exactly with every op on other
+    // qubits and can wait for a window that already holds its target
+    std::vector<std::pair<uint32_t, uint32_t>> deferred;
+    bool defer_cx = true;
     std::vector<unsigned char> win_data;    // 1q / dense matrices (copied into LDS)
     std::vector<unsigned char> win_tables;  // diagonal tables (read from global)
     // launch profiling: HIP events around every state-sweep launch
@@ -165,6 +170,7 @@ struct zb_backend {
 namespace {
 
 void flush(zb_backend *h);
+void flush_window(zb_backend *h);
 
 template <typename T>
 void launch_zero(zb_backend *h) {
@@ -394,7 +400,7 @@ void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint6
     if (!h->win_ops.empty() && ((uint64_t)__builtin_popcountll(h->win_mask | need) > m ||
                                 page_bits_of(h, h->win_mask | need) > h->page_bits ||
                                 desc_bytes(h->win_ops.size() + 1, h->win_data.size() + add) > dev::kDescBytes))
-        flush(h);
+        flush_window(h);
     if (h->win_ops.empty()) h->win_mask = low_mask(h);
     h->win_mask |= need;
     zb_backend::POp op{};
@@ -552,10 +558,43 @@ void flush_T(zb_backend *h) {
     h->win_tables.clear();
 }
 
-void flush(zb_backend *h) {
+// Launches the open window (if any) without touching held-back CX ops.
+void flush_window(zb_backend *h) {
     if (h->win_ops.empty()) return;
     if (h->prec == 64) flush_T<double>(h);
     else flush_T<float>(h);
+}
+
+template <typename T>
+void enqueue_cx(zb_backend *h, uint32_t c, uint32_t t) {
+    const uint32_t qs[2] = {t, c};  // bits[0] = target, bits[1] = control
+    enqueue<T>(h, dev::kWCx, qs, 2, 1ull << t, nullptr, 0);
+}
+
+// Queues, in order, the held-back CX ops that share a bit with `bits`.
+template <typename T>
+void settle(zb_backend *h, uint64_t bits) {
+    if (h->deferred.empty()) return;
+    std::vector<std::pair<uint32_t, uint32_t>> keep, emit;
+    for (const auto &cx : h->deferred)
+        (((bits >> cx.first) & 1) || ((bits >> cx.second) & 1) ? emit : keep).push_back(cx);
+    h->deferred.swap(keep);
+    for (const auto &cx : emit) enqueue_cx<T>(h, cx.first, cx.second);
+}
+
+// Everything queued reaches the device: held-back CX ops, then the window.
+void flush(zb_backend *h) {
+    if (h->prec == 64) settle<double>(h, ~0ull);
+    else settle<float>(h, ~0ull);
+    flush_window(h);
+}
+
+// The state is being overwritten: queued work is dropped.
+void discard_pending(zb_backend *h) {
+    h->win_ops.clear();
+    h->win_data.clear();
+    h->win_tables.clear();
+    h->deferred.clear();
 }
 
 // The apply_* entry points take circuit qubits and hand state bits to the
@@ -576,6 +615,7 @@ void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     }
     uint64_t need = 0;
     for (uint32_t b = 0; b < k; ++b) need |= 1ull << pq[b];
+    settle<T>(h, need);
     enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, pq, k, need, mat, (size_t)1 << (2 * k));
 }
 
@@ -584,6 +624,9 @@ void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     uint32_t pq[dev::kMaxDiagK];
     for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
     if (!tiled(h)) return exact_diag_T<T>(h, pq, k, d);
+    uint64_t bits = 0;
+    for (uint32_t b = 0; b < k; ++b) bits |= 1ull << pq[b];
+    settle<T>(h, bits);
     enqueue<T>(h, dev::kWDiag, pq, k, 0, d, (size_t)1 << k);
 }
 
@@ -592,8 +635,18 @@ void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
     c = h->phys[c];
     t = h->phys[t];
     if (!tiled(h)) return exact_cx_T<T>(h, c, t);
-    const uint32_t qs[2] = {t, c};  // bits[0] = target, bits[1] = control
-    enqueue<T>(h, dev::kWCx, qs, 2, 1ull << t, nullptr, 0);
+    settle<T>(h, (1ull << c) | (1ull << t));
+    const uint64_t mask = h->win_ops.empty() ? low_mask(h) : h->win_mask;
+    if (h->defer_cx && !((mask >> t) & 1)) {  // wait for a window that holds the target
+        if (h->deferred.size() >= 64) {
+            const auto oldest = h->deferred.front();
+            h->deferred.erase(h->deferred.begin());
+            enqueue_cx<T>(h, oldest.first, oldest.second);
+        }
+        h->deferred.emplace_back(c, t);
+        return;
+    }
+    enqueue_cx<T>(h, c, t);
 }
 
 // Fixed-order reduction (mode 0: norm, mode 1: prob_one of q).
@@ -720,9 +773,7 @@ void read_state_T(zb_backend *h, T *re, T *im) {
 
 template <typename T>
 void write_state_T(zb_backend *h, const T *re, const T *im) {
-    h->win_ops.clear();  // the state is being overwritten
-    h->win_data.clear();
-    h->win_tables.clear();
+    discard_pending(h);  // the state is being overwritten
     if (h->ident) {
         check(hipMemcpyAsync(h->re, re, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy re");
         check(hipMemcpyAsync(h->im, im, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy im");
@@ -844,6 +895,7 @@ int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
     else if (k == "slots" && (value == 0 || value == 3 || value == 4)) h->slots = (int)value;
     else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
     else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
+    else if (k == "defer_cx") h->defer_cx = value != 0;
     else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
         h->sync();
         set_layout(h, (int)value);
@@ -916,9 +968,7 @@ int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re, void *i
     }
     return guarded([&] {
         h->sync();  // work queued on the old state completes first
-        h->win_ops.clear();
-        h->win_data.clear();
-        h->win_tables.clear();
+        discard_pending(h);
         if (h->owns_state) {
             if (h->re) check(hipFree(h->re), "hipFree re");
             if (h->im) check(hipFree(h->im), "hipFree im");
@@ -1020,9 +1070,7 @@ int zb_measure(zb_backend *h, uint32_t q, double u) {
 
 int zb_reset_all(zb_backend *h) {
     return guarded([&] {
-        h->win_ops.clear();  // pending ops act on a state that is being overwritten
-        h->win_data.clear();
-        h->win_tables.clear();
+        discard_pending(h);  // pending ops act on a state that is being overwritten
         if (h->prec == 64) launch_zero<double>(h);
         else launch_zero<float>(h);
         h->sync();
@@ -1169,7 +1217,7 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
 }
 
 int zb_tune_default(uint32_t n_qubits, int precision, const char *key, int64_t value) {
-    static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits", "layout"};
+    static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits", "defer_cx", "layout"};
     const std::string k(key ? key : "");
     if (std::find(std::begin(keys), std::end(keys), k) == std::end(keys) || (precision != 32 && precision != 64)) {
         set_error("zb_tune_default: unknown key or precision: " + k);

@@ -6,9 +6,9 @@ and its bench can only compare fusion widths side by side
 chosen by measurement on the device at hand:
 
 * knobs that never change results — tile width (12/13 bits), register
-  slots per stage (3/4), register prefetch, window grid, qubit layout — are
-  searched by coordinate descent and installed as process-wide defaults
-  (zb_tune_default), so every backend of that shape created afterwards
+  slots per stage (3/4), register prefetch, window grid, CX deferral, qubit
+  layout — are searched by coordinate descent and installed as process-wide
+  defaults (zb_tune_default), so every backend of that shape created afterwards
   (run_compiled, the server, HipBackend) uses them;
 * the dense fusion width K changes the op stream and hence the rounding
   (still within the reference's tolerances, but no longer bit-identical to
@@ -37,6 +37,7 @@ SEARCH = {
     "slots": [0, 3, 4],
     "prefetch": [1, 0],
     "win_grid": [2048, 1024, 4096],
+    "defer_cx": [1, 0],
     "layout": [-1, 0],
 }
 FUSION = [1, 2, 3, 4]
