@@ -482,6 +482,13 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
             pi[j] = im[g];
         }
     }
+    // Write-back registers of the previous tile.  Its global stores are
+    // issued only after this tile's loads have landed in LDS: the vector
+    // memory counter retires in issue order, so a store issued before the
+    // wait on a prefetched load would be waited on with it.
+    T wr[R], wi[R];
+    uint64_t wbase = 0;
+    bool pending = false;
     while (tile < tiles) {
         if constexpr (!kPrefetch) {
 #pragma unroll
@@ -495,6 +502,14 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
         for (int j = 0; j < R; ++j) {
             sre[lt + (B + B / 16) * j] = pr[j];
             sim[lt + (B + B / 16) * j] = pi[j];
+        }
+        if (pending) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t g = row(wbase, j) | lo;
+                re[g] = wr[j];
+                im[g] = wi[j];
+            }
         }
         __syncthreads();
         const uint64_t next = tile + gridDim.x;
@@ -622,21 +637,24 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
         }
         // write-back through its own registers: pr/pi may already hold the
         // prefetched next tile
-        T wr[R], wi[R];
 #pragma unroll
         for (int j = 0; j < R; ++j) {
             wr[j] = sre[lt + (B + B / 16) * j];
             wi[j] = sim[lt + (B + B / 16) * j];
         }
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint64_t g = row(base, j) | lo;
-            re[g] = wr[j];
-            im[g] = wi[j];
-        }
+        wbase = base;
+        pending = true;
         __syncthreads();
         tile = next;
         base = nbase;
+    }
+    if (pending) {
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            const uint64_t g = row(wbase, j) | lo;
+            re[g] = wr[j];
+            im[g] = wi[j];
+        }
     }
 }
 
