@@ -175,7 +175,15 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * waits, bit-exactly, for a window that holds it; a CX is a permutation and
  * commutes with every op on other qubits) and
  * "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
- * resets the state to |0...0>).  None of them changes results. */
+ * resets the state to |0...0>).  None of them changes results, except
+ * "mfma" (0/1, default 0): dense ops on 4 to 6 qubits run as real GEMMs on
+ * the matrix cores (v_mfma_*_16x16x4) when n >= 12.  MFMA accumulation
+ * rounds differently from the CPU path's sum, so those ops agree with it
+ * within tolerance (max |d amp| <= 1e-12 in f64, <= 1e-5 in f32 on unit
+ * states) instead of bit for bit; and "fma" (0/1, default 0): every
+ * sweep's multiply-accumulate steps, in the CPU path's expression order,
+ * become fused multiply-adds (one rounding instead of two), with the same
+ * tolerance. */
 int zb_tune(zb_backend *h, const char *key, int64_t value);
 /* Process-wide default for a tuning knob of zb_tune, applied to every
  * backend of n_qubits at this precision when it is created or re-bound
@@ -196,6 +204,7 @@ int zb_tune_default(uint32_t n_qubits, int precision, const char *key,
 #define ZB_K_DIAGK 4
 #define ZB_K_WINDOW 5
 #define ZB_K_KQ 10 /* + k for the dense k-qubit kernel */
+#define ZB_K_KQ_MFMA 20 /* + k for the dense k-qubit kernel on matrix cores */
 int zb_profile_begin(zb_backend *h);
 int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms,
                        double *bytes, int64_t cap);

@@ -175,6 +175,107 @@ def test_adversarial_circuits_bit_exact(orc, mode):
                 _assert_same(g, o, f"n={n} fusion={fusion}")
 
 
+def _unitary(rng, k):
+    q, r = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
+    return q * (np.diag(r) / np.abs(np.diag(r)))
+
+
+def _max_diff(g, o):
+    gr, gi = g.raw()
+    orr, oi = o.raw()
+    return max(np.max(np.abs(gr.astype(np.float64) - orr)), np.max(np.abs(gi.astype(np.float64) - oi)))
+
+
+MFMA_TOL = {64: 1e-12, 32: 1e-5}
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_mfma_dense_ops_within_tolerance(orc, prec):
+    """Dense 4..6-qubit ops on the matrix cores (knob mfma=1) against the
+    oracle's scalar kern_kq on the same state: op qubits on the low run
+    bits, the top bits, mixed, under a shuffled qubit layout (op bits out of
+    order), and at 23 qubits where every workgroup walks several tiles."""
+    rng = np.random.default_rng(5)
+    tol = MFMA_TOL[prec]
+    for n, layout in ((12, 0), (13, 0), (16, 0), (16, 7)):
+        re, im = _rand_state(n, prec, 40 + n)
+        o = orc.State(n, prec)
+        o.set_raw(re, im)
+        with z.HipBackend(n, prec, z.HipBackend.EXACT) as g:
+            g.tune("layout", layout)
+            g.tune("mfma", 1)
+            g.set_raw(re, im)
+            for k in (4, 5, 6):
+                for qs in (list(range(k)), list(range(n - k, n)), sorted(rng.choice(n, k, replace=False).tolist())):
+                    m = _unitary(rng, k)
+                    _say(f"mfma prec={prec} n={n} layout={layout} k={k} qs={qs}")
+                    g.apply_unitary(qs, m)
+                    o.apply_unitary(qs, m)
+                    d = _max_diff(g, o)
+                    assert d <= tol, f"n={n} k={k} qs={qs}: max |d| = {d:.3e}"
+    n = 23
+    orc.set_threads(16)
+    try:
+        re, im = _rand_state(n, prec, 77)
+        o = orc.State(n, prec)
+        o.set_raw(re, im)
+        with z.HipBackend(n, prec, z.HipBackend.EXACT) as g:
+            g.tune("mfma", 1)
+            g.set_raw(re, im)
+            for k, qs in ((4, [0, 6, 13, 22]), (5, [1, 2, 3, 17, 20]), (6, [5, 8, 9, 10, 11, 21])):
+                m = _unitary(rng, k)
+                _say(f"mfma prec={prec} n={n} k={k} qs={qs}")
+                g.apply_unitary(qs, m)
+                o.apply_unitary(qs, m)
+            d = _max_diff(g, o)
+            assert d <= tol, f"n={n}: max |d| = {d:.3e}"
+    finally:
+        orc.set_threads(1)
+
+
+@pytest.mark.parametrize("prec", [64, 32])
+def test_mfma_fused_circuits_within_tolerance(orc, prec):
+    """Compiled circuits at fusion 4..6 with the dense blocks on the matrix
+    cores, in the tiled mode, against the oracle's scalar path."""
+    tol = MFMA_TOL[prec]
+    for n in (12, 14, 17):
+        for prog in (orc.gen_random(n, 10, 300 + n), orc.gen_matching(n, 10, 310 + n, 3)):
+            for fusion in (4, 5, 6):
+                _say(f"mfma circuit prec={prec} n={n} fusion={fusion}")
+                o = orc.State(n, prec)
+                o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+                with z.HipBackend(n, prec, z.HipBackend.TILED) as g:
+                    g.tune("mfma", 1)
+                    g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                    d = _max_diff(g, o)
+                    assert d <= tol, f"n={n} fusion={fusion}: max |d| = {d:.3e}"
+
+
+@pytest.mark.parametrize("mode", MODES)
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fma_knob_within_tolerance(orc, prec, mode):
+    """Knob fma=1: the same expression trees with fused multiply-adds, in
+    the per-op kernels and the tiled windows, against the oracle's separately
+    rounded path.  The results must differ somewhere (the knob is live) and
+    stay within tolerance."""
+    tol = MFMA_TOL[prec]
+    worst = 0.0
+    for n in (5, 12, 14, 17):
+        for prog in (orc.gen_random(n, 12, 500 + n), orc.gen_matching(n, 12, 510 + n, 3)):
+            for fusion, knobs in ((1, {"fma": 1}), (2, {"fma": 1}), (4, {"fma": 1}), (5, {"fma": 1, "mfma": 1})):
+                _say(f"fma prec={prec} mode={mode} n={n} fusion={fusion} knobs={knobs}")
+                o = orc.State(n, prec)
+                o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+                with z.HipBackend(n, prec, mode) as g:
+                    for k, v in knobs.items():
+                        g.tune(k, v)
+                    g.apply_compiled(z.compile(prog, n, fusion=fusion))
+                    d = _max_diff(g, o)
+                    assert d <= tol, f"n={n} fusion={fusion}: max |d| = {d:.3e}"
+                    worst = max(worst, d)
+    assert worst > 0.0
+
+
 def test_sampler_indices_and_chunk_sums_bit_exact(orc):
     for n, shots, seed in ((1, 100, 3), (4, 1000, 1), (10, 5000, 2), (16, 20000, 5), (17, 4096, 3), (20, 100000, 4)):
         _say(f"sample n={n} shots={shots}")

@@ -15,7 +15,8 @@ CSRC = os.path.join(_HERE, "csrc")
 
 ZB_OK, ZB_ERR_ARG, ZB_ERR_UNSUPPORTED, ZB_ERR_MEMORY, ZB_ERR_DEVICE = 0, -1, -2, -3, -4
 KERNEL_NAMES = {0: "k_zero", 1: "k_1q", 2: "k_cx", 3: "k_diag1", 4: "k_diagk", 5: "k_window",
-                **{10 + k: f"k_kq<{k}>" for k in range(2, 7)}}
+                **{10 + k: f"k_kq<{k}>" for k in range(2, 7)},
+                **{20 + k: f"k_kq_mfma<{k}>" for k in range(4, 7)}}
 
 # Every exported symbol of include/zeno_hip.h with its ctypes signature.
 P, U64, I64, I32, U32, D = C.c_void_p, C.c_uint64, C.c_int64, C.c_int, C.c_uint32, C.c_double

@@ -90,6 +90,8 @@ struct zb_backend {
     // qubits and can wait for a window that already holds its target
     std::vector<std::pair<uint32_t, uint32_t>> deferred;
     bool defer_cx = true;
+    bool mfma = false;  // dense 4..6-qubit ops on the matrix cores (within tolerance)
+    bool fma = false;   // fused multiply-adds in the sweep arithmetic (within tolerance)
     std::vector<unsigned char> win_data;    // 1q / dense matrices (copied into LDS)
     std::vector<unsigned char> win_tables;  // diagonal tables (read from global)
     // launch profiling: HIP events around every state-sweep launch
@@ -256,10 +258,60 @@ template <typename T, int K>
 void launch_kq(zb_backend *h, const T *dmat, const dev::Qubits &qs) {
     const uint64_t groups = h->len >> K;
     h->timed(ZB_K_KQ + K, h->sweep_bytes(), [&] {
-        hipLaunchKernelGGL((dev::k_kq<T, K>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                           h->I<T>(), dmat, qs, groups);
+        if (h->fma)
+            hipLaunchKernelGGL((dev::k_kq<T, K, true>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream,
+                               h->R<T>(), h->I<T>(), dmat, qs, groups);
+        else
+            hipLaunchKernelGGL((dev::k_kq<T, K, false>), dim3(grid_for(groups)), dim3(dev::kBlock), 0, h->stream,
+                               h->R<T>(), h->I<T>(), dmat, qs, groups);
     });
     check(hipGetLastError(), "k_kq");
+}
+
+// Tile of the matrix-core kernel for an op on state bits pq (op order):
+// the op's bits, the low bits of a 128-byte run, then the lowest free bits,
+// ascending; X row bit b <-> op bit b, the other tile bits index groups.
+dev::MfmaTile mfma_tile(const zb_backend *h, const uint32_t *pq, uint32_t k) {
+    uint64_t m = 0;
+    for (uint32_t b = 0; b < k; ++b) m |= 1ull << pq[b];
+    const uint32_t run = h->prec == 64 ? 4 : 5;
+    m |= (1ull << run) - 1;
+    for (uint32_t b = 0; __builtin_popcountll(m) < dev::kMfmaTileBits; ++b) m |= 1ull << b;
+    dev::MfmaTile t{};
+    uint32_t i = 0, g = 0;
+    for (uint32_t b = 0; b < 64; ++b) {
+        if (!((m >> b) & 1)) continue;
+        t.tq[i] = b;
+        t.role[i] = g;
+        for (uint32_t j = 0; j < k; ++j)
+            if (pq[j] == b) t.role[i] = dev::kRoleOp | j;
+        if (!(t.role[i] & dev::kRoleOp)) ++g;
+        ++i;
+    }
+    return t;
+}
+
+template <typename T, int K>
+void launch_kq_mfma(zb_backend *h, const std::vector<T> &mr, const std::vector<T> &mi, const uint32_t *pq) {
+    constexpr int D = 1 << K, D2 = 2 * D;
+    std::vector<T> w((size_t)D2 * D2);
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) {
+            const T a = mr[r * D + c], b = mi[r * D + c];
+            w[r * D2 + c] = a;
+            w[r * D2 + D + c] = -b;
+            w[(D + r) * D2 + c] = b;
+            w[(D + r) * D2 + D + c] = a;
+        }
+    const T *dw = static_cast<const T *>(h->upload(w.data(), w.size() * sizeof(T)));
+    const dev::MfmaTile t = mfma_tile(h, pq, K);
+    const uint64_t tiles = h->len >> dev::kMfmaTileBits;
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, 1024);
+    h->timed(ZB_K_KQ_MFMA + K, h->sweep_bytes(), [&] {
+        hipLaunchKernelGGL((dev::k_kq_mfma<T, K>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream, h->R<T>(),
+                           h->I<T>(), dw, t, tiles);
+    });
+    check(hipGetLastError(), "k_kq_mfma");
 }
 
 template <typename T>
@@ -267,13 +319,24 @@ void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     const size_t dim = (size_t)1 << k;
     std::vector<T> mr, mi;
     to_T<T>(mat, dim * dim, mr, mi);
+    if (h->mfma && k >= 4 && h->n >= (uint32_t)dev::kMfmaTileBits) {
+        switch (k) {
+            case 4: return launch_kq_mfma<T, 4>(h, mr, mi, qs);
+            case 5: return launch_kq_mfma<T, 5>(h, mr, mi, qs);
+            case 6: return launch_kq_mfma<T, 6>(h, mr, mi, qs);
+        }
+    }
     if (k == 1) {
         dev::Mat2<T> m;
         for (int i = 0; i < 4; ++i) m.r[i] = mr[i], m.i[i] = mi[i];
         const uint64_t half = h->len >> 1;
         h->timed(ZB_K_1Q, h->sweep_bytes(), [&] {
-            hipLaunchKernelGGL(dev::k_1q<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                               h->I<T>(), qs[0], half, m);
+            if (h->fma)
+                hipLaunchKernelGGL((dev::k_1q<T, true>), dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream,
+                                   h->R<T>(), h->I<T>(), qs[0], half, m);
+            else
+                hipLaunchKernelGGL((dev::k_1q<T, false>), dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream,
+                                   h->R<T>(), h->I<T>(), qs[0], half, m);
         });
         check(hipGetLastError(), "k_1q");
         return;
@@ -300,8 +363,12 @@ void exact_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     if (k == 1) {
         const uint64_t half = h->len >> 1;
         h->timed(ZB_K_DIAG1, h->sweep_bytes(), [&] {
-            hipLaunchKernelGGL(dev::k_diag1<T>, dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                               h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
+            if (h->fma)
+                hipLaunchKernelGGL((dev::k_diag1<T, true>), dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream,
+                                   h->R<T>(), h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
+            else
+                hipLaunchKernelGGL((dev::k_diag1<T, false>), dim3(grid_for(half)), dim3(dev::kBlock), 0, h->stream,
+                                   h->R<T>(), h->I<T>(), qs[0], half, dr[0], di[0], dr[1], di[1]);
         });
         check(hipGetLastError(), "k_diag1");
         return;
@@ -312,8 +379,12 @@ void exact_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     const T *tab = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
     const dev::Qubits q = make_qubits(qs, k);
     h->timed(ZB_K_DIAGK, h->sweep_bytes(), [&] {
-        hipLaunchKernelGGL(dev::k_diagk<T>, dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                           h->I<T>(), tab, q, h->len);
+        if (h->fma)
+            hipLaunchKernelGGL((dev::k_diagk<T, true>), dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream,
+                               h->R<T>(), h->I<T>(), tab, q, h->len);
+        else
+            hipLaunchKernelGGL((dev::k_diagk<T, false>), dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream,
+                               h->R<T>(), h->I<T>(), tab, q, h->len);
     });
     check(hipGetLastError(), "k_diagk");
 }
@@ -365,18 +436,25 @@ template <typename T, int TB, int NS>
 void launch_window_shape(zb_backend *h, bool dense, unsigned grid, const uint32_t *blob, const T *tab,
                          uint64_t tiles) {
     const dim3 g(grid), b(dev::WinShape<T, TB, NS>::kLanes);
-    if (dense && h->prefetch)
-        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
-                           tab, tiles);
+    // the fused-multiply-add variants exist with register prefetch only
+    if (h->fma && dense)
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(),
+                           blob, tab, tiles);
+    else if (h->fma)
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, true, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(),
+                           blob, tab, tiles);
+    else if (dense && h->prefetch)
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(),
+                           blob, tab, tiles);
     else if (dense)
-        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
-                           tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, true, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(),
+                           blob, tab, tiles);
     else if (h->prefetch)
-        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, true>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
-                           tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, true, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(),
+                           blob, tab, tiles);
     else
-        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, false>), g, b, 0, h->stream, h->R<T>(), h->I<T>(), blob,
-                           tab, tiles);
+        hipLaunchKernelGGL((dev::k_window<T, TB, NS, false, false, false>), g, b, 0, h->stream, h->R<T>(),
+                           h->I<T>(), blob, tab, tiles);
 }
 
 template <typename T>
@@ -887,8 +965,9 @@ void zero_state(zb_backend *h) {
     h->sync();
 }
 
-// One tuning knob (none changes results); ZB_ERR_ARG for an unknown key
-// or value.  "layout" resets the state to |0...0>.
+// One tuning knob (none but "mfma" and "fma" changes results, and those
+// only within tolerance); ZB_ERR_ARG for an unknown key or value.  "layout"
+// resets the state to |0...0>.
 int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
     if (k == "prefetch") h->prefetch = value != 0;
     else if (k == "win_grid" && value >= 1 && value <= (1 << 20)) h->win_grid = (unsigned)value;
@@ -896,6 +975,8 @@ int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
     else if (k == "page_bits" && value >= 1 && value <= 12) h->page_bits = (uint32_t)value;
     else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
     else if (k == "defer_cx") h->defer_cx = value != 0;
+    else if (k == "mfma") h->mfma = value != 0;
+    else if (k == "fma") h->fma = value != 0;
     else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
         h->sync();
         set_layout(h, (int)value);
@@ -1217,7 +1298,8 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
 }
 
 int zb_tune_default(uint32_t n_qubits, int precision, const char *key, int64_t value) {
-    static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits", "defer_cx", "layout"};
+    static const char *keys[] = {"prefetch", "win_grid", "slots",  "page_bits", "tile_bits",
+                                 "defer_cx", "layout",   "mfma", "fma"};
     const std::string k(key ? key : "");
     if (std::find(std::begin(keys), std::end(keys), k) == std::end(keys) || (precision != 32 && precision != 64)) {
         set_error("zb_tune_default: unknown key or precision: " + k);

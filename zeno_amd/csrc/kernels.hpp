@@ -4,7 +4,10 @@
 // in HBM.  Every kernel below reproduces the reference CPU path's
 // expression tree for its op (src/state.rs), and the library is compiled
 // with -ffp-contract=off, so no a*b+c is contracted to an FMA: results are
-// bit-identical to the scalar CPU path for the same inputs.
+// bit-identical to the scalar CPU path for the same inputs.  Two opt-in
+// knobs trade that for speed within tolerance: "fma" (fused multiply-adds
+// in the same expression order) and "mfma" (dense 4..6-qubit ops on the
+// matrix cores).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -55,6 +58,24 @@ __device__ __forceinline__ uint64_t phys_index(const Layout &L, uint64_t l) {
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
 
+// Multiply-accumulate steps of the sweep arithmetic, in the CPU path's
+// expression order.  F = false is the CPU path's own rounding: a separately
+// rounded product and sum (the library is built with -ffp-contract=off, so
+// acc + a*b is never fused).  F = true (knob "fma") rounds once per step
+// with a fused multiply-add: fewer VALU operations, not bit-identical.
+template <bool F, typename T>
+__device__ __forceinline__ T mac(T acc, T a, T b) {
+    if constexpr (!F) return acc + a * b;
+    else if constexpr (sizeof(T) == 4) return __builtin_fmaf(a, b, acc);
+    else return __builtin_fma(a, b, acc);
+}
+template <bool F, typename T>
+__device__ __forceinline__ T msc(T acc, T a, T b) {
+    if constexpr (!F) return acc - a * b;
+    else if constexpr (sizeof(T) == 4) return __builtin_fmaf(-a, b, acc);
+    else return __builtin_fma(-a, b, acc);
+}
+
 // ------------------------------------------------------------- zero state
 // StateVec::zero_state / reset_zero (src/state.rs:109-132).
 template <typename T>
@@ -67,17 +88,17 @@ __global__ __launch_bounds__(kBlock) void k_zero(T *__restrict__ re, T *__restri
 
 // -------------------------------------------------------------- 1q dense
 // kern_1q_scalar (src/state.rs:232-260): pair (a, a + 2^q) per half index.
-template <typename T>
+template <typename T, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_1q(T *__restrict__ re, T *__restrict__ im, uint32_t q, uint64_t half,
                                                Mat2<T> m) {
     const uint64_t s = 1ull << q;
     for (uint64_t h = gtid(); h < half; h += gstride()) {
         const uint64_t a = insert_zero(h, q), b = a + s;
         const T ar = re[a], ai = im[a], br = re[b], bi = im[b];
-        re[a] = m.r[0] * ar - m.i[0] * ai + m.r[1] * br - m.i[1] * bi;
-        im[a] = m.r[0] * ai + m.i[0] * ar + m.r[1] * bi + m.i[1] * br;
-        re[b] = m.r[2] * ar - m.i[2] * ai + m.r[3] * br - m.i[3] * bi;
-        im[b] = m.r[2] * ai + m.i[2] * ar + m.r[3] * bi + m.i[3] * br;
+        re[a] = msc<F>(mac<F>(msc<F>(m.r[0] * ar, m.i[0], ai), m.r[1], br), m.i[1], bi);
+        im[a] = mac<F>(mac<F>(mac<F>(m.r[0] * ai, m.i[0], ar), m.r[1], bi), m.i[1], br);
+        re[b] = msc<F>(mac<F>(msc<F>(m.r[2] * ar, m.i[2], ai), m.r[3], br), m.i[3], bi);
+        im[b] = mac<F>(mac<F>(mac<F>(m.r[2] * ai, m.i[2], ar), m.r[3], bi), m.i[3], br);
     }
 }
 
@@ -86,7 +107,7 @@ __global__ __launch_bounds__(kBlock) void k_1q(T *__restrict__ re, T *__restrict
 // row-major matvec with ar = (ar + wr*xr) - wi*xi, ai = (ai + wr*xi) + wi*xr
 // over ascending columns, scatter.  The matrix ([re | im], 2*4^K of T) is
 // staged in LDS; every lane reads the same element (broadcast).
-template <typename T, int K>
+template <typename T, int K, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict__ im, const T *__restrict__ mat,
                                                Qubits qs, uint64_t groups) {
     constexpr int D = 1 << K;
@@ -117,8 +138,8 @@ __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict
             const T *wr = sm + r * D, *wi = sm + D * D + r * D;
 #pragma unroll
             for (int c = 0; c < D; ++c) {
-                ar = ar + wr[c] * xr[c] - wi[c] * xi[c];
-                ai = ai + wr[c] * xi[c] + wi[c] * xr[c];
+                ar = msc<F>(mac<F>(ar, wr[c], xr[c]), wi[c], xi[c]);
+                ai = mac<F>(mac<F>(ai, wr[c], xi[c]), wi[c], xr[c]);
             }
             uint64_t o = base;
 #pragma unroll
@@ -126,6 +147,167 @@ __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict
                 if ((r >> b) & 1) o |= bit[b];
             re[o] = ar;
             im[o] = ai;
+        }
+    }
+}
+
+// ------------------------------------------------------ kq on matrix cores
+// The same op as k_kq for K = 4..6, as a real GEMM on the matrix cores
+// (opt-in: MFMA accumulation does not round like kern_kq_scalar's sum, so
+// results agree with the CPU path within tolerance, not bit for bit).
+// With D = 2^K, y = M x is [yr; yi] = [[Mr, -Mi], [Mi, Mr]] [xr; xi], a
+// 2D x 2D real matrix W.  A workgroup loads a tile of 2^12 amplitudes that
+// holds the op's K state bits, so the tile is G = 2^(12-K) whole groups,
+// and stores it in LDS as X[2D][G] (row j < D: re of amplitude j of every
+// group, row D + j: its im).  Y = W X is 32 output blocks of 16 x 16
+// (v_mfma_*_16x16x4: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k; D[i][j]
+// in lane j + 16(i/4), element i%4 for f32, lane j + 16(i%4), element i/4 for
+// f64); each of the 8 waves computes 4 blocks of one row band, holding that
+// band of W in registers for all tiles.
+constexpr int kMfmaTileBits = 12;
+constexpr int kMfmaThreads = 512;
+constexpr uint32_t kRoleOp = 0x100;  // role[i]: kRoleOp | b for op bit b, else group bit index
+
+struct MfmaTile {
+    uint32_t tq[kMfmaTileBits];    // local bit i <-> state bit tq[i] (ascending)
+    uint32_t role[kMfmaTileBits];  // what local bit i indexes in X
+};
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+
+template <typename T>
+struct MfmaOps;
+template <>
+struct MfmaOps<float> {
+    using Acc = f32x4;
+    static __device__ __forceinline__ Acc step(float a, float b, Acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // row of a 16 x 16 output block held by element v of lane ln
+    static __device__ __forceinline__ int row(int ln, int v) { return 4 * (ln >> 4) + v; }
+};
+template <>
+struct MfmaOps<double> {
+    using Acc = f64x4;
+    static __device__ __forceinline__ Acc step(double a, double b, Acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    static __device__ __forceinline__ int row(int ln, int v) { return (ln >> 4) + 4 * v; }
+};
+
+template <typename T, int K>
+__global__ __launch_bounds__(kMfmaThreads) void k_kq_mfma(T *__restrict__ re, T *__restrict__ im,
+                                                          const T *__restrict__ w, MfmaTile mt, uint64_t tiles) {
+    using Ops = MfmaOps<T>;
+    using Acc = typename Ops::Acc;
+    constexpr int TB = kMfmaTileBits, R = (1 << TB) / kMfmaThreads, LB = TB - 3;
+    constexpr int D = 1 << K, D2 = 2 * D, G = 1 << (TB - K), SP = G + 16;
+    constexpr int CB = G / 16, KS = D2 / 4;
+    static_assert(R == 8 && (32 * 16 * 16) == D2 * G, "tile shape");
+    __shared__ T sx[D2 * SP];
+    const int t = threadIdx.x, wv = t >> 6, ln = t & 63;
+    const int rb = (wv * 4) / CB, cb0 = (wv * 4) % CB;
+    T a[KS];
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) a[ks] = w[(16 * rb + (ln & 15)) * D2 + 4 * ks + (ln >> 4)];
+    // lane part and register part of a tile element's state offset and X slot
+    uint64_t lo = 0;
+    int lofs = 0;
+#pragma unroll
+    for (int i = 0; i < LB; ++i) {
+        if (!((t >> i) & 1)) continue;
+        lo |= 1ull << mt.tq[i];
+        const uint32_t ro = mt.role[i];
+        lofs += (ro & kRoleOp) ? (1 << (ro & 7)) * SP : (1 << ro);
+    }
+    uint64_t hi[R];
+    int rofs[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        hi[r] = 0;
+        rofs[r] = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            if (!((r >> i) & 1)) continue;
+            hi[r] |= 1ull << mt.tq[LB + i];
+            const uint32_t ro = mt.role[LB + i];
+            rofs[r] += (ro & kRoleOp) ? (1 << (ro & 7)) * SP : (1 << ro);
+        }
+    }
+    auto tile_base = [&](uint64_t tile) {
+        uint64_t b = tile;
+#pragma unroll
+        for (int i = 0; i < TB; ++i) b = insert_zero(b, mt.tq[i]);
+        return b;
+    };
+    T pr[R], pi[R], wr[R], wi[R];
+    uint64_t tile = blockIdx.x, base = tile < tiles ? tile_base(tile) : 0, wbase = 0;
+    bool pending = false;
+    if (tile < tiles) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            pr[r] = re[base | hi[r] | lo];
+            pi[r] = im[base | hi[r] | lo];
+        }
+    }
+    while (tile < tiles) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            sx[lofs + rofs[r]] = pr[r];
+            sx[lofs + rofs[r] + D * SP] = pi[r];
+        }
+        if (pending) {  // previous tile's stores, after this tile's loads landed
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                re[wbase | hi[r] | lo] = wr[r];
+                im[wbase | hi[r] | lo] = wi[r];
+            }
+        }
+        __syncthreads();
+        const uint64_t next = tile + gridDim.x;
+        const uint64_t nbase = next < tiles ? tile_base(next) : 0;
+        if (next < tiles) {
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                pr[r] = re[nbase | hi[r] | lo];
+                pi[r] = im[nbase | hi[r] | lo];
+            }
+        }
+        Acc acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = Acc{0, 0, 0, 0};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const T b = sx[(4 * ks + (ln >> 4)) * SP + 16 * (cb0 + i) + (ln & 15)];
+                acc[i] = Ops::step(a[ks], b, acc[i]);
+            }
+        }
+        __syncthreads();  // every wave has read X
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                sx[(16 * rb + Ops::row(ln, v)) * SP + 16 * (cb0 + i) + (ln & 15)] = acc[i][v];
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            wr[r] = sx[lofs + rofs[r]];
+            wi[r] = sx[lofs + rofs[r] + D * SP];
+        }
+        wbase = base;
+        pending = true;
+        __syncthreads();
+        tile = next;
+        base = nbase;
+    }
+    if (pending) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            re[wbase | hi[r] | lo] = wr[r];
+            im[wbase | hi[r] | lo] = wi[r];
         }
     }
 }
@@ -149,23 +331,23 @@ __global__ __launch_bounds__(kBlock) void k_cx(T *__restrict__ re, T *__restrict
 // -------------------------------------------------------------- diagonal
 // kern_diag1_scalar (src/state.rs:498-527): d[0..1] on the a side, d[2..3]
 // on the b side, re = wr*ar - wi*ai, im = wr*ai + wi*ar.
-template <typename T>
+template <typename T, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_diag1(T *__restrict__ re, T *__restrict__ im, uint32_t q, uint64_t half,
                                                   T d0r, T d0i, T d1r, T d1i) {
     const uint64_t s = 1ull << q;
     for (uint64_t h = gtid(); h < half; h += gstride()) {
         const uint64_t a = insert_zero(h, q), b = a + s;
         const T ar = re[a], ai = im[a], br = re[b], bi = im[b];
-        re[a] = d0r * ar - d0i * ai;
-        im[a] = d0r * ai + d0i * ar;
-        re[b] = d1r * br - d1i * bi;
-        im[b] = d1r * bi + d1i * br;
+        re[a] = msc<F>(d0r * ar, d0i, ai);
+        im[a] = mac<F>(d0r * ai, d0i, ar);
+        re[b] = msc<F>(d1r * br, d1i, bi);
+        im[b] = mac<F>(d1r * bi, d1i, br);
     }
 }
 
 // kern_diagk_scalar (src/state.rs:577-596): table index j gathered from the
 // bits of i at qs; table ([re | im], 2*2^k of T) staged in LDS.
-template <typename T>
+template <typename T, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_diagk(T *__restrict__ re, T *__restrict__ im, const T *__restrict__ tab,
                                                   Qubits qs, uint64_t len) {
     __shared__ T st[2 << kMaxDiagK];
@@ -180,8 +362,8 @@ __global__ __launch_bounds__(kBlock) void k_diagk(T *__restrict__ re, T *__restr
         for (int b = 0; b < qs.k; ++b) j |= (uint32_t)((i >> qs.q[b]) & 1) << b;
         const T wr = st[j], wi = st[(1 << kMaxDiagK) + j];
         const T ar = re[i], ai = im[i];
-        re[i] = wr * ar - wi * ai;
-        im[i] = wr * ai + wi * ar;
+        re[i] = msc<F>(wr * ar, wi, ai);
+        im[i] = mac<F>(wr * ai, wi, ar);
     }
 }
 
@@ -300,7 +482,7 @@ struct WinHeader {
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
 
-template <typename T, int NS, int S>
+template <typename T, int NS, int S, bool F>
 __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
     const T m00r = d[0], m01r = d[1], m10r = d[2], m11r = d[3];
     const T m00i = d[4], m01i = d[5], m10i = d[6], m11i = d[7];
@@ -309,15 +491,15 @@ __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
         if ((r >> S) & 1) continue;
         const int a = r, c = r | (1 << S);
         const T ar = xr[a], ai = xi[a], br = xr[c], bi = xi[c];
-        xr[a] = m00r * ar - m00i * ai + m01r * br - m01i * bi;
-        xi[a] = m00r * ai + m00i * ar + m01r * bi + m01i * br;
-        xr[c] = m10r * ar - m10i * ai + m11r * br - m11i * bi;
-        xi[c] = m10r * ai + m10i * ar + m11r * bi + m11i * br;
+        xr[a] = msc<F>(mac<F>(msc<F>(m00r * ar, m00i, ai), m01r, br), m01i, bi);
+        xi[a] = mac<F>(mac<F>(mac<F>(m00r * ai, m00i, ar), m01r, bi), m01i, br);
+        xr[c] = msc<F>(mac<F>(msc<F>(m10r * ar, m10i, ai), m11r, br), m11i, bi);
+        xi[c] = mac<F>(mac<F>(mac<F>(m10r * ai, m10i, ar), m11r, bi), m11i, br);
     }
 }
 
 // Dense op on the slots of mask M (matrix local bit b <-> b-th set bit).
-template <typename T, int NS, int M>
+template <typename T, int NS, int M, bool F>
 __device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d) {
     constexpr int K = __builtin_popcount(M);
     constexpr int D = 1 << K;
@@ -347,8 +529,8 @@ __device__ __forceinline__ void reg_dense(T *xr, T *xi, const T *__restrict__ d)
 #pragma unroll
             for (int c = 0; c < D; ++c) {
                 const T wr = d[row * D + c], wi = d[D * D + row * D + c];
-                ar = ar + wr * vr[c] - wi * vi[c];
-                ai = ai + wr * vi[c] + wi * vr[c];
+                ar = msc<F>(mac<F>(ar, wr, vr[c]), wi, vi[c]);
+                ai = mac<F>(mac<F>(ai, wr, vi[c]), wi, vr[c]);
             }
             yr[row] = ar;
             yi[row] = ai;
@@ -393,25 +575,25 @@ __device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
     }
 }
 
-template <typename T, int NS, int M>
+template <typename T, int NS, int M, bool F>
 __device__ __forceinline__ void reg_dense_if(T *xr, T *xi, const T *__restrict__ d) {
-    if constexpr (M < (1 << NS)) reg_dense<T, NS, M>(xr, xi, d);
+    if constexpr (M < (1 << NS)) reg_dense<T, NS, M, F>(xr, xi, d);
 }
 
-template <typename T, int NS>
+template <typename T, int NS, bool F>
 __device__ __forceinline__ void reg_dense_dispatch(T *xr, T *xi, uint32_t mask, const T *__restrict__ d) {
     switch (mask) {
-        case 0x3: reg_dense_if<T, NS, 0x3>(xr, xi, d); break;
-        case 0x5: reg_dense_if<T, NS, 0x5>(xr, xi, d); break;
-        case 0x6: reg_dense_if<T, NS, 0x6>(xr, xi, d); break;
-        case 0x7: reg_dense_if<T, NS, 0x7>(xr, xi, d); break;
-        case 0x9: reg_dense_if<T, NS, 0x9>(xr, xi, d); break;
-        case 0xA: reg_dense_if<T, NS, 0xA>(xr, xi, d); break;
-        case 0xC: reg_dense_if<T, NS, 0xC>(xr, xi, d); break;
-        case 0xB: reg_dense_if<T, NS, 0xB>(xr, xi, d); break;
-        case 0xD: reg_dense_if<T, NS, 0xD>(xr, xi, d); break;
-        case 0xE: reg_dense_if<T, NS, 0xE>(xr, xi, d); break;
-        case 0xF: reg_dense_if<T, NS, 0xF>(xr, xi, d); break;
+        case 0x3: reg_dense_if<T, NS, 0x3, F>(xr, xi, d); break;
+        case 0x5: reg_dense_if<T, NS, 0x5, F>(xr, xi, d); break;
+        case 0x6: reg_dense_if<T, NS, 0x6, F>(xr, xi, d); break;
+        case 0x7: reg_dense_if<T, NS, 0x7, F>(xr, xi, d); break;
+        case 0x9: reg_dense_if<T, NS, 0x9, F>(xr, xi, d); break;
+        case 0xA: reg_dense_if<T, NS, 0xA, F>(xr, xi, d); break;
+        case 0xC: reg_dense_if<T, NS, 0xC, F>(xr, xi, d); break;
+        case 0xB: reg_dense_if<T, NS, 0xB, F>(xr, xi, d); break;
+        case 0xD: reg_dense_if<T, NS, 0xD, F>(xr, xi, d); break;
+        case 0xE: reg_dense_if<T, NS, 0xE, F>(xr, xi, d); break;
+        case 0xF: reg_dense_if<T, NS, 0xF, F>(xr, xi, d); break;
     }
 }
 
@@ -438,7 +620,7 @@ struct WinShape {
 // keep the register budget of the common 1q/CX/diagonal windows low).
 // Global addressing: row j of a tile starts at the uniform element offset
 // base | hi(j); a lane adds its offset lo (its low local bits).
-template <typename T, int TB, int NS, bool kDense, bool kPrefetch>
+template <typename T, int TB, int NS, bool kDense, bool kPrefetch, bool F>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, WinShape<T, TB, NS>::kLanes)))
 __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_window(
     T *__restrict__ re, T *__restrict__ im, const uint32_t *__restrict__ blob, const T *__restrict__ tables,
@@ -552,15 +734,15 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                     case kW1q: {
                         const T *d = mats + op->data_off;
                         switch (sel) {
-                            case 0: reg_1q<T, NS, 0>(xr, xi, d); break;
-                            case 1: reg_1q<T, NS, 1>(xr, xi, d); break;
-                            case 2: if constexpr (NS > 2) reg_1q<T, NS, 2>(xr, xi, d); break;
-                            case 3: if constexpr (NS > 3) reg_1q<T, NS, 3>(xr, xi, d); break;
+                            case 0: reg_1q<T, NS, 0, F>(xr, xi, d); break;
+                            case 1: reg_1q<T, NS, 1, F>(xr, xi, d); break;
+                            case 2: if constexpr (NS > 2) reg_1q<T, NS, 2, F>(xr, xi, d); break;
+                            case 3: if constexpr (NS > 3) reg_1q<T, NS, 3, F>(xr, xi, d); break;
                         }
                         break;
                     }
                     case kWDense:
-                        if constexpr (kDense) reg_dense_dispatch<T, NS>(xr, xi, sel, mats + op->data_off);
+                        if constexpr (kDense) reg_dense_dispatch<T, NS, F>(xr, xi, sel, mats + op->data_off);
                         break;
                     case kWDiag: {
                         // table index: bits fixed by the tile or by the lane, plus the
@@ -589,8 +771,8 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                             for (int i = 0; i < NS; ++i) j |= ((r >> i) & 1) ? js[i] : 0u;
                             const T wr = d[j], wi = d[D + j];
                             const T ar = xr[r], ai = xi[r];
-                            xr[r] = wr * ar - wi * ai;
-                            xi[r] = wr * ai + wi * ar;
+                            xr[r] = msc<F>(wr * ar, wi, ai);
+                            xi[r] = mac<F>(wr * ai, wi, ar);
                         }
                         break;
                     }
