@@ -41,14 +41,14 @@ def effective_bytes(ops, n, s):
 TUNE = {}
 
 
-def run_gpu(n, comp, prec, mode, steps, warmup):
+def run_gpu(n, comp, prec, mode, steps, warmup, knobs=None):
     """Time `steps` full circuits (zero init + every op + finish) after
     `warmup` untimed ones; kernel durations come from HIP events the library
     records around each launch on its own stream."""
     import zeno_amd as z
 
     be = z.HipBackend(n, prec, mode)
-    for k, v in TUNE.items():
+    for k, v in dict(TUNE, **(knobs or {})).items():
         be.tune(k, v)
     step_times, recs = [], []
     for step in range(warmup + steps):
@@ -219,6 +219,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=48, help="serving pool in GiB")
     ap.add_argument("--no-extra", action="store_true", help="skip the trajectory and serving legs")
+    ap.add_argument("--no-variants", action="store_true",
+                    help="skip the opt-in fma / mfma variants (within tolerance, not bit-identical)")
     args = ap.parse_args()
     for kv in [x for x in args.tune.split(",") if x]:
         k, v = kv.split("=")
@@ -255,6 +257,30 @@ def main():
     err = be32.max_abs_diff(be64)
     norm64 = be64.norm_sqr()
     be32.close()
+
+    # Opt-in knobs that trade bit-identity with the CPU path for speed,
+    # each timed once and checked against the exact f64 state above.
+    variants = {}
+    if not args.no_variants:
+        comp5 = None
+        for label, prec, fusion, knobs in (("f64_fma", 64, args.fusion, {"fma": 1}),
+                                           ("f32_fma", 32, args.fusion, {"fma": 1}),
+                                           ("f32_fusion5_mfma", 32, 5, {"mfma": 1}),
+                                           ("f64_fusion5_mfma", 64, 5, {"mfma": 1}),
+                                           ("f32_fusion5_valu", 32, 5, {})):
+            if fusion == args.fusion:
+                cv = comp
+            else:
+                comp5 = comp5 or z.compile(prog, n, fusion=fusion)
+                cv = comp5
+            _log(f"variant {label} ...")
+            bv, tv, kv, byv = run_gpu(n, cv, prec, args.mode, 1, 1, knobs)
+            variants[label] = {"precision": f"f{prec}", "fusion": fusion, "knobs": knobs, "ops": len(cv),
+                               "s_per_circuit": round(tv, 4), "speedup_vs_f64_exact": round(s64 / tv, 3),
+                               "max_abs_damp_vs_f64_exact": bv.max_abs_diff(be64),
+                               "kernels": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)}
+                                           for k, v in kv.items()}}
+            bv.close()
     be64.close()
 
     alternatives = []
@@ -320,6 +346,7 @@ def main():
         "kernels_f64": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls64.items()},
         "kernels_f32": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls32.items()},
         "alternatives": alternatives,
+        "variants": variants,
         "roofline": {
             "bound": "hbm",
             "kernel": dom_cls,
