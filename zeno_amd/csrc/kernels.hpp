@@ -730,81 +730,67 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
             for (int o = (int)st->op_begin; o < oe; ++o) {
                 const WinOp *op = ops + o;
                 const uint32_t kind = op->kind, sel = op->sel;
-                switch (kind) {
-                    case kW1q: {
-                        const T *d = mats + op->data_off;
-                        switch (sel) {
-                            case 0: reg_1q<T, NS, 0, F>(xr, xi, d); break;
-                            case 1: reg_1q<T, NS, 1, F>(xr, xi, d); break;
-                            case 2: if constexpr (NS > 2) reg_1q<T, NS, 2, F>(xr, xi, d); break;
-                            case 3: if constexpr (NS > 3) reg_1q<T, NS, 3, F>(xr, xi, d); break;
-                        }
-                        break;
-                    }
-                    case kWDense:
-                        if constexpr (kDense) reg_dense_dispatch<T, NS, F>(xr, xi, sel, mats + op->data_off);
-                        break;
-                    case kWDiag: {
-                        // table index: bits fixed by the tile or by the lane, plus the
-                        // bits that vary across the lane's registers
-                        const T *d = tables + op->data_off;
-                        const int k = (int)op->k, D = 1 << k;
-                        uint32_t jt = 0, js[NS];
+                // one flat dispatch over (kind, sel): nested switches merge the
+                // register state twice and the merges cost register copies
+                constexpr uint32_t k1 = kW1q << 8, kc = kWCx << 8, ks = kCtrlSlot << 4, kl = kCtrlLocal << 4,
+                                   kf = kCtrlFixed << 4;
+                const bool on = (sel >> 4) == kCtrlLocal ? ((lb >> op->aux) & 1) : ((base >> op->aux) & 1);
+                switch (kind << 8 | sel) {
+                    case k1 | 0: reg_1q<T, NS, 0, F>(xr, xi, mats + op->data_off); break;
+                    case k1 | 1: reg_1q<T, NS, 1, F>(xr, xi, mats + op->data_off); break;
+                    case k1 | 2: if constexpr (NS > 2) reg_1q<T, NS, 2, F>(xr, xi, mats + op->data_off); break;
+                    case k1 | 3: if constexpr (NS > 3) reg_1q<T, NS, 3, F>(xr, xi, mats + op->data_off); break;
+                    case kc | ks | 0 | 1 << 2: reg_cx_slot<T, NS, 0, 1>(xr, xi); break;
+                    case kc | ks | 0 | 2 << 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 0, 2>(xr, xi); break;
+                    case kc | ks | 0 | 3 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 0, 3>(xr, xi); break;
+                    case kc | ks | 1 | 0 << 2: reg_cx_slot<T, NS, 1, 0>(xr, xi); break;
+                    case kc | ks | 1 | 2 << 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 1, 2>(xr, xi); break;
+                    case kc | ks | 1 | 3 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 1, 3>(xr, xi); break;
+                    case kc | ks | 2 | 0 << 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 0>(xr, xi); break;
+                    case kc | ks | 2 | 1 << 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 1>(xr, xi); break;
+                    case kc | ks | 2 | 3 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 2, 3>(xr, xi); break;
+                    case kc | ks | 3 | 0 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 0>(xr, xi); break;
+                    case kc | ks | 3 | 1 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 1>(xr, xi); break;
+                    case kc | ks | 3 | 2 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 2>(xr, xi); break;
+                    case kc | kl | 0: case kc | kf | 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
+                    case kc | kl | 1: case kc | kf | 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
+                    case kc | kl | 2: case kc | kf | 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
+                    case kc | kl | 3: case kc | kf | 3: if constexpr (NS > 3) reg_cx_all<T, NS, 3>(xr, xi, on); break;
+                    default:
+                        if (kind == kWDense) {
+                            if constexpr (kDense) reg_dense_dispatch<T, NS, F>(xr, xi, sel, mats + op->data_off);
+                        } else if (kind == kWDiag) {
+                            // table index: bits fixed by the tile or by the lane, plus the
+                            // bits that vary across the lane's registers
+                            const T *d = tables + op->data_off;
+                            const int k = (int)op->k, D = 1 << k;
+                            uint32_t jt = 0, js[NS];
 #pragma unroll
-                        for (int i = 0; i < NS; ++i) js[i] = 0;
-                        for (int b = 0; b < k; ++b) {
-                            const uint32_t bb = op->bits[b];
-                            if (bb & kFixedBit) {
-                                jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
-                                continue;
+                            for (int i = 0; i < NS; ++i) js[i] = 0;
+                            for (int b = 0; b < k; ++b) {
+                                const uint32_t bb = op->bits[b];
+                                if (bb & kFixedBit) {
+                                    jt |= (uint32_t)((base >> (bb & ~kFixedBit)) & 1) << b;
+                                    continue;
+                                }
+                                bool slot = false;
+#pragma unroll
+                                for (int i = 0; i < NS; ++i)
+                                    if ((int)bb == sl[i]) js[i] = 1u << b, slot = true;
+                                if (!slot) jt |= (uint32_t)((lb >> bb) & 1) << b;
                             }
-                            bool slot = false;
 #pragma unroll
-                            for (int i = 0; i < NS; ++i)
-                                if ((int)bb == sl[i]) js[i] = 1u << b, slot = true;
-                            if (!slot) jt |= (uint32_t)((lb >> bb) & 1) << b;
-                        }
+                            for (int r = 0; r < R; ++r) {
+                                uint32_t j = jt;
 #pragma unroll
-                        for (int r = 0; r < R; ++r) {
-                            uint32_t j = jt;
-#pragma unroll
-                            for (int i = 0; i < NS; ++i) j |= ((r >> i) & 1) ? js[i] : 0u;
-                            const T wr = d[j], wi = d[D + j];
-                            const T ar = xr[r], ai = xi[r];
-                            xr[r] = msc<F>(wr * ar, wi, ai);
-                            xi[r] = mac<F>(wr * ai, wi, ar);
-                        }
-                        break;
-                    }
-                    case kWCx: {
-                        const uint32_t st_ = sel & 3, sc = (sel >> 2) & 3, mode = sel >> 4;
-                        if (mode == kCtrlSlot) {
-                            switch (st_ * 4 + sc) {
-                                case 1: reg_cx_slot<T, NS, 0, 1>(xr, xi); break;
-                                case 2: if constexpr (NS > 2) reg_cx_slot<T, NS, 0, 2>(xr, xi); break;
-                                case 3: if constexpr (NS > 3) reg_cx_slot<T, NS, 0, 3>(xr, xi); break;
-                                case 4: reg_cx_slot<T, NS, 1, 0>(xr, xi); break;
-                                case 6: if constexpr (NS > 2) reg_cx_slot<T, NS, 1, 2>(xr, xi); break;
-                                case 7: if constexpr (NS > 3) reg_cx_slot<T, NS, 1, 3>(xr, xi); break;
-                                case 8: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 0>(xr, xi); break;
-                                case 9: if constexpr (NS > 2) reg_cx_slot<T, NS, 2, 1>(xr, xi); break;
-                                case 11: if constexpr (NS > 3) reg_cx_slot<T, NS, 2, 3>(xr, xi); break;
-                                case 12: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 0>(xr, xi); break;
-                                case 13: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 1>(xr, xi); break;
-                                case 14: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 2>(xr, xi); break;
-                            }
-                        } else {
-                            const uint32_t aux = op->aux;
-                            const bool on = mode == kCtrlLocal ? ((lb >> aux) & 1) : ((base >> aux) & 1);
-                            switch (st_) {
-                                case 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
-                                case 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
-                                case 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
-                                case 3: if constexpr (NS > 3) reg_cx_all<T, NS, 3>(xr, xi, on); break;
+                                for (int i = 0; i < NS; ++i) j |= ((r >> i) & 1) ? js[i] : 0u;
+                                const T wr = d[j], wi = d[D + j];
+                                const T ar = xr[r], ai = xi[r];
+                                xr[r] = msc<F>(wr * ar, wi, ai);
+                                xi[r] = mac<F>(wr * ai, wi, ar);
                             }
                         }
                         break;
-                    }
                 }
             }
 #pragma unroll
