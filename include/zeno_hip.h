@@ -167,8 +167,8 @@ int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms);
 int zb_layout(const zb_backend *h, uint32_t *phys);
 /* Tuning knobs of the tiled kernel (flushes first): "prefetch" (0/1:
  * issue the next tile's HBM loads into registers while the current tile
- * computes), "win_grid" (workgroups per window launch), "slots" (3 or 4
- * register qubits per stage, 0 = by precision) and "page_bits" (1..12:
+ * computes), "win_grid" (workgroups per window launch, default 16384),
+ * "slots" (3 or 4 register qubits per stage, 0 = 3) and "page_bits" (1..12:
  * most window qubits that select a 2 MiB page of the state, default 6),
  * "tile_bits" (12 or 13: tile width of the tiled kernel, 0 = 12),
  * "defer_cx" (0/1, default 1: a CX whose target is outside the open window

@@ -62,8 +62,8 @@ struct zb_backend {
     hipStream_t stream = nullptr;
     int mode = 1;
     bool prefetch = true;  // tiled kernel variant: register prefetch of the next tile
-    int slots = 0;          // tiled stage width: 3 (512-lane workgroups) or 4 (256 lanes); 0 = by precision
-    unsigned win_grid = 2048;
+    int slots = 0;          // tiled stage width: 3 (512-lane workgroups) or 4 (256 lanes); 0 = 3
+    unsigned win_grid = 16384;  // workgroups per window launch (at most one per tile)
     uint32_t page_bits = 6;  // most tile qubits at or above the page bit in one window
     uint32_t tile_bits = 0;  // tile width of the tiled kernel: 12 or 13 (0 = 12)
     std::string name;
@@ -430,7 +430,7 @@ size_t desc_bytes(size_t nops, size_t data) {
     return pad(sizeof(dev::WinHeader)) + pad(nops * sizeof(dev::WinStage)) + pad(nops * sizeof(dev::WinOp)) + data;
 }
 
-int nslots(const zb_backend *h) { return h->slots ? h->slots : (h->prec == 64 ? 3 : 4); }
+int nslots(const zb_backend *h) { return h->slots ? h->slots : 3; }
 
 template <typename T, int TB, int NS>
 void launch_window_shape(zb_backend *h, bool dense, unsigned grid, const uint32_t *blob, const T *tab,

@@ -34,9 +34,9 @@ from .backend import HipBackend, _check
 # Searched in this order (coordinate descent from the library defaults).
 SEARCH = {
     "tile_bits": [12, 13],
-    "slots": [0, 3, 4],
+    "slots": [0, 4],
     "prefetch": [1, 0],
-    "win_grid": [2048, 1024, 4096],
+    "win_grid": [16384, 4096, 65536],
     "defer_cx": [1, 0],
     "layout": [-1, 0],
 }
