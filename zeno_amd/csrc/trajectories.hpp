@@ -84,18 +84,40 @@ struct TrajCtx {
     const T *pauli;  // LDS: X then Y as [re 4 | im 4]
 };
 
+// The state of a trajectory lives in L2 / the last-level cache; a sweep is
+// bound by the latency of those loads.  Each lane therefore loads a batch
+// of kTrajBatch<T> independent pairs before computing and storing any of
+// them (the loop order does not change any pair's arithmetic).
+template <typename T>
+constexpr int kTrajBatch = sizeof(T) == 4 ? 8 : 4;
+
 template <typename T>
 __device__ __forceinline__ void t_1q(const TrajCtx<T> c, uint32_t q, const T *m) {
     const T m00r = m[0], m01r = m[1], m10r = m[2], m11r = m[3];
     const T m00i = m[4], m01i = m[5], m10i = m[6], m11i = m[7];
-    const uint64_t s = 1ull << q;
-    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
-        const uint64_t a = insert_zero(h, q), b = a + s;
-        const T ar = c.re[a], ai = c.im[a], br = c.re[b], bi = c.im[b];
-        c.re[a] = m00r * ar - m00i * ai + m01r * br - m01i * bi;
-        c.im[a] = m00r * ai + m00i * ar + m01r * bi + m01i * br;
-        c.re[b] = m10r * ar - m10i * ai + m11r * br - m11i * bi;
-        c.im[b] = m10r * ai + m10i * ar + m11r * bi + m11i * br;
+    constexpr int U = kTrajBatch<T>;
+    const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
+    for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
+        T ar[U], ai[U], br[U], bi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t a = insert_zero(h, q), b = a + s;
+                ar[u] = c.re[a], ai[u] = c.im[a], br[u] = c.re[b], bi[u] = c.im[b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t a = insert_zero(h, q), b = a + s;
+                c.re[a] = m00r * ar[u] - m00i * ai[u] + m01r * br[u] - m01i * bi[u];
+                c.im[a] = m00r * ai[u] + m00i * ar[u] + m01r * bi[u] + m01i * br[u];
+                c.re[b] = m10r * ar[u] - m10i * ai[u] + m11r * br[u] - m11i * bi[u];
+                c.im[b] = m10r * ai[u] + m10i * ar[u] + m11r * bi[u] + m11i * br[u];
+            }
+        }
     }
     __syncthreads();
 }
@@ -142,14 +164,29 @@ __device__ __forceinline__ void t_kq(const TrajCtx<T> c, const uint32_t *qs, con
 // kern_diag1_scalar (src/state.rs:498-527): d0 on the a side, d1 on the b side.
 template <typename T>
 __device__ __forceinline__ void t_diag1(const TrajCtx<T> c, uint32_t q, T d0r, T d1r, T d0i, T d1i) {
-    const uint64_t s = 1ull << q;
-    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
-        const uint64_t a = insert_zero(h, q), b = a + s;
-        const T ar = c.re[a], ai = c.im[a], br = c.re[b], bi = c.im[b];
-        c.re[a] = d0r * ar - d0i * ai;
-        c.im[a] = d0r * ai + d0i * ar;
-        c.re[b] = d1r * br - d1i * bi;
-        c.im[b] = d1r * bi + d1i * br;
+    constexpr int U = kTrajBatch<T>;
+    const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
+    for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
+        T ar[U], ai[U], br[U], bi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t a = insert_zero(h, q), b = a + s;
+                ar[u] = c.re[a], ai[u] = c.im[a], br[u] = c.re[b], bi[u] = c.im[b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t a = insert_zero(h, q), b = a + s;
+                c.re[a] = d0r * ar[u] - d0i * ai[u];
+                c.im[a] = d0r * ai[u] + d0i * ar[u];
+                c.re[b] = d1r * br[u] - d1i * bi[u];
+                c.im[b] = d1r * bi[u] + d1i * br[u];
+            }
+        }
     }
     __syncthreads();
 }
@@ -162,14 +199,28 @@ __device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, i
         t_diag1(c, qs[0], d[0], d[1], d[2], d[3]);
         return;
     } else {
-        for (uint64_t i = threadIdx.x; i < c.len; i += blockDim.x) {
-            uint32_t j = 0;
+        constexpr int U = kTrajBatch<T>;
+        const uint64_t step = (uint64_t)blockDim.x;
+        for (uint64_t i0 = threadIdx.x; i0 < c.len; i0 += U * step) {
+            T ar[U], ai[U];
 #pragma unroll
-            for (int b = 0; b < kTrajMaxK; ++b)
-                if (b < k) j |= (uint32_t)((i >> qs[b]) & 1) << b;
-            const T wr = d[j], wi = d[D + j], ar = c.re[i], ai = c.im[i];
-            c.re[i] = wr * ar - wi * ai;
-            c.im[i] = wr * ai + wi * ar;
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + u * step;
+                if (i < c.len) ar[u] = c.re[i], ai[u] = c.im[i];
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t i = i0 + u * step;
+                if (i < c.len) {
+                    uint32_t j = 0;
+#pragma unroll
+                    for (int b = 0; b < kTrajMaxK; ++b)
+                        if (b < k) j |= (uint32_t)((i >> qs[b]) & 1) << b;
+                    const T wr = d[j], wi = d[D + j];
+                    c.re[i] = wr * ar[u] - wi * ai[u];
+                    c.im[i] = wr * ai[u] + wi * ar[u];
+                }
+            }
         }
     }
     __syncthreads();
@@ -178,13 +229,29 @@ __device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, i
 template <typename T>
 __device__ __forceinline__ void t_cx(const TrajCtx<T> c, uint32_t ctl, uint32_t tgt) {
     const uint32_t lo = ctl < tgt ? ctl : tgt, hi = ctl < tgt ? tgt : ctl;
-    for (uint64_t g = threadIdx.x; g < (c.len >> 2); g += blockDim.x) {
-        const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
-        const T r0 = c.re[a], r1 = c.re[b], i0 = c.im[a], i1 = c.im[b];
-        c.re[a] = r1;
-        c.re[b] = r0;
-        c.im[a] = i1;
-        c.im[b] = i0;
+    constexpr int U = kTrajBatch<T>;
+    const uint64_t quarter = c.len >> 2, step = (uint64_t)blockDim.x;
+    for (uint64_t g0 = threadIdx.x; g0 < quarter; g0 += U * step) {
+        T r0[U], r1[U], i0[U], i1[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t g = g0 + u * step;
+            if (g < quarter) {
+                const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
+                r0[u] = c.re[a], r1[u] = c.re[b], i0[u] = c.im[a], i1[u] = c.im[b];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t g = g0 + u * step;
+            if (g < quarter) {
+                const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
+                c.re[a] = r1[u];
+                c.re[b] = r0[u];
+                c.im[a] = i1[u];
+                c.im[b] = i0[u];
+            }
+        }
     }
     __syncthreads();
 }
@@ -244,14 +311,30 @@ __device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, uint32_t q) {
 
 template <typename T>
 __device__ __forceinline__ void t_collapse(const TrajCtx<T> c, uint32_t q, int outcome, T scale) {
-    const uint64_t s = 1ull << q;
-    for (uint64_t h = threadIdx.x; h < (c.len >> 1); h += blockDim.x) {
-        const uint64_t a = insert_zero(h, q);
-        const uint64_t keep = outcome ? a + s : a, zero = outcome ? a : a + s;
-        c.re[keep] = c.re[keep] * scale;
-        c.im[keep] = c.im[keep] * scale;
-        c.re[zero] = T(0);
-        c.im[zero] = T(0);
+    constexpr int U = kTrajBatch<T>;
+    const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
+    for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
+        T kr[U], ki[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t keep = insert_zero(h, q) + (outcome ? s : 0);
+                kr[u] = c.re[keep], ki[u] = c.im[keep];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const uint64_t h = h0 + u * step;
+            if (h < half) {
+                const uint64_t a = insert_zero(h, q);
+                const uint64_t keep = outcome ? a + s : a, zero = outcome ? a : a + s;
+                c.re[keep] = kr[u] * scale;
+                c.im[keep] = ki[u] * scale;
+                c.re[zero] = T(0);
+                c.im[zero] = T(0);
+            }
+        }
     }
     __syncthreads();
 }
