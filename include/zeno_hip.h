@@ -173,7 +173,11 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * "tile_bits" (12 or 13: tile width of the tiled kernel, 0 = 12),
  * "defer_cx" (0/1, default 1: a CX whose target is outside the open window
  * waits, bit-exactly, for a window that holds it; a CX is a permutation and
- * commutes with every op on other qubits) and
+ * commutes with every op on other qubits), "relabel" (0/1, default 1: when
+ * ops arrive through zr_apply_ops or a run, a window that closes because
+ * the next op does not fit also moves onto the low state bits, on
+ * write-back, the qubits of its tile that the coming ops need soonest; a
+ * relabelling permutes state index bits only) and
  * "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
  * resets the state to |0...0>).  None of them changes results, except
  * "mfma" (0/1, default 0): dense ops on 4 to 6 qubits run as real GEMMs on

@@ -175,6 +175,32 @@ def test_adversarial_circuits_bit_exact(orc, mode):
                 _assert_same(g, o, f"n={n} fusion={fusion}")
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_relabelling_windows_bit_exact(orc, prec):
+    """Batched runs relabel qubits on window write-back (the layout moves);
+    the state, norm, probabilities and samples equal the oracle's and a run
+    with relabelling off, bit for bit."""
+    for n, prog, fusion in ((20, orc.gen_matching(20, 12, 41, 5), 1), (22, orc.gen_random(22, 10, 42), 2),
+                            (24, orc.gen_matching(24, 8, 43, 3), 1)):
+        _say(f"relabel prec={prec} n={n} fusion={fusion}")
+        o = orc.State(n, prec)
+        o.apply_compiled(orc.Compiled(prog, n, fusion=fusion))
+        comp = z.compile(prog, n, fusion=fusion)
+        with z.HipBackend(n, prec, z.HipBackend.TILED) as g, z.HipBackend(n, prec, z.HipBackend.TILED) as f:
+            start = list(g.layout())
+            f.tune("relabel", 0)
+            g.apply_compiled(comp)
+            f.apply_compiled(comp)
+            assert list(g.layout()) != start, "no window relabelled"
+            assert list(f.layout()) == start
+            _assert_same(g, o, f"n={n} relabel")
+            _assert_same(f, o, f"n={n} no relabel")
+            assert g.norm_sqr() == o.norm_sqr()
+            for q in (0, n // 2, n - 1):
+                assert g.prob_one(q) == o.prob_one(q)
+            assert np.array_equal(g.sample(5000, 3), o.sample(5000, 3))
+
+
 def _unitary(rng, k):
     q, r = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
     return q * (np.diag(r) / np.abs(np.diag(r)))

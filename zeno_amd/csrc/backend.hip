@@ -88,8 +88,19 @@ struct zb_backend {
     // CX ops held back (physical control, target; pairwise disjoint): a CX is a
     // pure permutation, so it commutes bit-exactly with every op on other
     // qubits and can wait for a window that already holds its target
-    std::vector<std::pair<uint32_t, uint32_t>> deferred;
+    std::vector<std::pair<uint32_t, uint32_t>> deferred;  // circuit qubits (control, target)
     bool defer_cx = true;
+    // Relabelling: when the caller has announced the coming ops (their
+    // tile needs as masks of circuit qubits), a window that closes because
+    // an op does not fit also permutes its tile's qubits on write-back, so
+    // that the low state bits every window holds go to the tile's qubits
+    // needed soonest.  The layout LUT then lags behind phys[] until the
+    // next reduction or copy refreshes it.
+    bool relabel = true;
+    const uint64_t *la_need = nullptr;
+    size_t la_count = 0, la_pos = 0;
+    std::vector<std::pair<uint32_t, uint32_t>> wb_swaps;  // state-bit swaps of the window being flushed
+    bool lut_dirty = false;
     bool mfma = false;  // dense 4..6-qubit ops on the matrix cores (within tolerance)
     bool fma = false;   // fused multiply-adds in the sweep arithmetic (within tolerance)
     std::vector<unsigned char> win_data;    // 1q / dense matrices (copied into LDS)
@@ -198,6 +209,8 @@ uint32_t page_bit(const zb_backend *h) { return h->prec == 64 ? 18 : 19; }
 // Results do not depend on the layout: every op acts on the same
 // amplitudes with the same arithmetic, and every reduction or copy walks
 // logical indices.  Set only while the state is |0...0> (index 0 is fixed).
+void upload_layout(zb_backend *h);
+
 void set_layout(zb_backend *h, int kind) {
     const uint32_t n = h->n, low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, n), pb = page_bit(h);
     if (kind < 0) kind = n >= pb + 4 ? 1 : 0;
@@ -218,6 +231,13 @@ void set_layout(zb_backend *h, int kind) {
         }
     }
     h->layout_kind = kind;
+    upload_layout(h);
+}
+
+// Uploads the LUT of phys[] (index -> state index), synchronously.
+void upload_layout(zb_backend *h) {
+    const uint32_t n = h->n;
+    h->lut_dirty = false;
     h->ident = true;
     for (uint32_t q = 0; q < n; ++q) h->ident = h->ident && h->phys[q] == q;
     if (h->ident) return;
@@ -469,16 +489,42 @@ void launch_window(zb_backend *h, int ns, bool dense, unsigned grid, const uint3
     }
 }
 
+void close_window(zb_backend *h);
 template <typename T>
-void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint64_t need, const double *src,
-             size_t n_complex) {
+void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat);
+
+// Queues an op on circuit qubits lq into the open window.  Its state bits
+// are taken from phys[] now, and again if closing the window relabelled.
+template <typename T>
+void enqueue(zb_backend *h, uint32_t kind, const uint32_t *lq, uint32_t k, const double *src, size_t n_complex) {
     const uint64_t m = tile_bits(h);
     const bool table = kind == dev::kWDiag;
     const size_t add = table ? 0 : 2 * n_complex * sizeof(T);
+    uint32_t qs[dev::kMaxDiagK];
+    uint64_t need = 0;
+    auto map = [&] {
+        need = 0;
+        for (uint32_t b = 0; b < k; ++b) qs[b] = h->phys[lq[b]];
+        if (kind == dev::kW1q || kind == dev::kWDense)
+            for (uint32_t b = 0; b < k; ++b) need |= 1ull << qs[b];
+        else if (kind == dev::kWCx)
+            need = 1ull << qs[0];  // the target; the control may stay outside the tile
+    };
+    map();
     if (!h->win_ops.empty() && ((uint64_t)__builtin_popcountll(h->win_mask | need) > m ||
                                 page_bits_of(h, h->win_mask | need) > h->page_bits ||
-                                desc_bytes(h->win_ops.size() + 1, h->win_data.size() + add) > dev::kDescBytes))
-        flush_window(h);
+                                desc_bytes(h->win_ops.size() + 1, h->win_data.size() + add) > dev::kDescBytes)) {
+        close_window(h);
+        map();
+    }
+    if (kind == dev::kWDense) {  // a relabelling may have put its qubits out of order
+        bool ascending = true;
+        for (uint32_t b = 1; b < k; ++b) ascending = ascending && qs[b] > qs[b - 1];
+        if (!ascending) {
+            flush_window(h);
+            return exact_unitary_T<T>(h, qs, k, src);
+        }
+    }
     if (h->win_ops.empty()) h->win_mask = low_mask(h);
     h->win_mask |= need;
     zb_backend::POp op{};
@@ -500,6 +546,7 @@ void enqueue(zb_backend *h, uint32_t kind, const uint32_t *qs, uint32_t k, uint6
 template <typename T>
 void flush_T(zb_backend *h) {
     if (h->win_ops.size() == 1) {  // one op: the per-op kernel is the cheaper pass
+        h->wb_swaps.clear();
         const zb_backend::POp op = h->win_ops[0];
         std::vector<double> inter;
         if (op.kind != dev::kWCx) {  // a CX carries no matrix
@@ -528,6 +575,10 @@ void flush_T(zb_backend *h) {
     uint32_t idx = 0;
     for (uint32_t q = 0; q < h->n; ++q)
         if ((mask >> q) & 1) hdr.q[idx] = q, local[q] = (int)idx++;
+    for (uint32_t i = 0; i < m; ++i) hdr.wb[i] = i;
+    for (const auto &sw : h->wb_swaps) std::swap(hdr.wb[local[sw.first]], hdr.wb[local[sw.second]]);
+    hdr.relabel = h->wb_swaps.empty() ? 0 : 1;
+    h->wb_swaps.clear();
     // Stages: greedy over the ops in order; a stage's slots must hold every
     // local bit its ops act on (CX: the target only; diagonals: none).
     auto needs = [&](const zb_backend::POp &p) {
@@ -643,19 +694,75 @@ void flush_window(zb_backend *h) {
     else flush_T<float>(h);
 }
 
-template <typename T>
-void enqueue_cx(zb_backend *h, uint32_t c, uint32_t t) {
-    const uint32_t qs[2] = {t, c};  // bits[0] = target, bits[1] = control
-    enqueue<T>(h, dev::kWCx, qs, 2, 1ull << t, nullptr, 0);
+// Closes the open window because the next op does not fit.  With the
+// coming ops known, the window's write-back also moves onto the low state
+// bits (which every window holds) those of its tile's qubits that the
+// coming ops need soonest, swapping them with the low bits' present
+// occupants; phys[] follows.  A relabelling is only a permutation of the
+// state index bits, applied to every amplitude of every tile: results do
+// not change.
+void close_window(zb_backend *h) {
+    h->wb_swaps.clear();
+    std::vector<std::pair<uint32_t, uint32_t>> moves;  // circuit qubits (into low, out of low)
+    if (h->relabel && h->la_need && h->win_ops.size() > 1) {
+        const uint32_t low = (uint32_t)__builtin_popcountll(low_mask(h));
+        uint64_t in_tile = 0;  // circuit qubits of the tile
+        for (uint32_t q = 0; q < h->n; ++q)
+            if ((h->win_mask >> h->phys[q]) & 1) in_tile |= 1ull << q;
+        std::vector<std::pair<size_t, uint32_t>> next;  // (next use, qubit)
+        uint64_t open = in_tile;
+        for (size_t i = h->la_pos; i < h->la_count && open; ++i) {
+            uint64_t hit = h->la_need[i] & open;
+            open &= ~hit;
+            for (; hit; hit &= hit - 1) next.emplace_back(i, (uint32_t)__builtin_ctzll(hit));
+        }
+        for (uint64_t r = open; r; r &= r - 1) next.emplace_back(SIZE_MAX, (uint32_t)__builtin_ctzll(r));
+        // soonest first; on ties the present low occupants stay
+        std::stable_sort(next.begin(), next.end(), [&](const auto &a, const auto &b) {
+            if (a.first != b.first) return a.first < b.first;
+            return (h->phys[a.second] < low) > (h->phys[b.second] < low);
+        });
+        uint64_t keep = 0;
+        for (size_t i = 0; i < next.size() && i < low; ++i) keep |= 1ull << next[i].second;
+        std::vector<uint32_t> in, out;
+        for (uint32_t q = 0; q < h->n; ++q) {
+            const bool is_low = h->phys[q] < low, kept = (keep >> q) & 1;
+            if (kept && !is_low) in.push_back(q);
+            if (!kept && is_low) out.push_back(q);
+        }
+        for (size_t i = 0; i < in.size() && i < out.size(); ++i) {
+            moves.emplace_back(in[i], out[i]);
+            h->wb_swaps.emplace_back(h->phys[in[i]], h->phys[out[i]]);
+        }
+    }
+    flush_window(h);
+    for (const auto &mv : moves) std::swap(h->phys[mv.first], h->phys[mv.second]);
+    if (!moves.empty()) h->lut_dirty = true;
 }
 
-// Queues, in order, the held-back CX ops that share a bit with `bits`.
+// Brings the device copy of the layout up to date with phys[] before a
+// kernel that walks circuit-order indices.
+void refresh_layout(zb_backend *h) {
+    if (!h->lut_dirty) return;
+    h->sync();
+    upload_layout(h);
+}
+
+// CX on circuit qubits (control c, target t).
 template <typename T>
-void settle(zb_backend *h, uint64_t bits) {
+void enqueue_cx(zb_backend *h, uint32_t c, uint32_t t) {
+    const uint32_t lq[2] = {t, c};  // bits[0] = target, bits[1] = control
+    enqueue<T>(h, dev::kWCx, lq, 2, nullptr, 0);
+}
+
+// Queues, in order, the held-back CX ops that share a qubit with `qubits`
+// (a mask of circuit qubits).
+template <typename T>
+void settle(zb_backend *h, uint64_t qubits) {
     if (h->deferred.empty()) return;
     std::vector<std::pair<uint32_t, uint32_t>> keep, emit;
     for (const auto &cx : h->deferred)
-        (((bits >> cx.first) & 1) || ((bits >> cx.second) & 1) ? emit : keep).push_back(cx);
+        (((qubits >> cx.first) & 1) || ((qubits >> cx.second) & 1) ? emit : keep).push_back(cx);
     h->deferred.swap(keep);
     for (const auto &cx : emit) enqueue_cx<T>(h, cx.first, cx.second);
 }
@@ -683,39 +790,40 @@ template <typename T>
 void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
     uint32_t pq[dev::kMaxDiagK];
     bool ascending = true;
+    uint64_t qubits = 0;
     for (uint32_t b = 0; b < k; ++b) {
         pq[b] = h->phys[qs[b]];
         ascending = ascending && (b == 0 || pq[b] > pq[b - 1]);
+        qubits |= 1ull << qs[b];
     }
     if (!tiled(h) || k > (uint32_t)nslots(h) || !ascending) {
-        flush(h);
+        flush(h);  // held-back CX ops may close a relabelling window: map again
+        for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
         return exact_unitary_T<T>(h, pq, k, mat);
     }
-    uint64_t need = 0;
-    for (uint32_t b = 0; b < k; ++b) need |= 1ull << pq[b];
-    settle<T>(h, need);
-    enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, pq, k, need, mat, (size_t)1 << (2 * k));
+    settle<T>(h, qubits);
+    enqueue<T>(h, k == 1 ? dev::kW1q : dev::kWDense, qs, k, mat, (size_t)1 << (2 * k));
 }
 
 template <typename T>
 void apply_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d) {
-    uint32_t pq[dev::kMaxDiagK];
-    for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
-    if (!tiled(h)) return exact_diag_T<T>(h, pq, k, d);
-    uint64_t bits = 0;
-    for (uint32_t b = 0; b < k; ++b) bits |= 1ull << pq[b];
-    settle<T>(h, bits);
-    enqueue<T>(h, dev::kWDiag, pq, k, 0, d, (size_t)1 << k);
+    if (!tiled(h)) {
+        uint32_t pq[dev::kMaxDiagK];
+        for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
+        return exact_diag_T<T>(h, pq, k, d);
+    }
+    uint64_t qubits = 0;
+    for (uint32_t b = 0; b < k; ++b) qubits |= 1ull << qs[b];
+    settle<T>(h, qubits);
+    enqueue<T>(h, dev::kWDiag, qs, k, d, (size_t)1 << k);
 }
 
 template <typename T>
 void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
-    c = h->phys[c];
-    t = h->phys[t];
-    if (!tiled(h)) return exact_cx_T<T>(h, c, t);
+    if (!tiled(h)) return exact_cx_T<T>(h, h->phys[c], h->phys[t]);
     settle<T>(h, (1ull << c) | (1ull << t));
     const uint64_t mask = h->win_ops.empty() ? low_mask(h) : h->win_mask;
-    if (h->defer_cx && !((mask >> t) & 1)) {  // wait for a window that holds the target
+    if (h->defer_cx && !((mask >> h->phys[t]) & 1)) {  // wait for a window that holds the target
         if (h->deferred.size() >= 64) {
             const auto oldest = h->deferred.front();
             h->deferred.erase(h->deferred.begin());
@@ -731,6 +839,7 @@ void apply_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
 template <typename T>
 double reduce_T(zb_backend *h, int mode, uint32_t q) {
     flush(h);
+    refresh_layout(h);
     uint64_t n = mode ? h->len >> 1 : h->len;
     uint64_t nb = (n + dev::kReduceSpan - 1) / dev::kReduceSpan;
     h->ensure_red(2 * nb + 2);
@@ -777,6 +886,7 @@ void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
 template <typename T>
 void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
     flush(h);
+    refresh_layout(h);
     if (shots == 0) return;
     const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
     const uint64_t nch = (h->len + chunk - 1) / chunk;
@@ -830,6 +940,7 @@ constexpr uint64_t kCopyChunk = 1ull << 22;
 template <typename T>
 void read_state_T(zb_backend *h, T *re, T *im) {
     flush(h);
+    refresh_layout(h);
     if (h->ident) {
         check(hipMemcpyAsync(re, h->re, h->len * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy re");
         check(hipMemcpyAsync(im, h->im, h->len * sizeof(T), hipMemcpyDeviceToHost, h->stream), "copy im");
@@ -852,6 +963,7 @@ void read_state_T(zb_backend *h, T *re, T *im) {
 template <typename T>
 void write_state_T(zb_backend *h, const T *re, const T *im) {
     discard_pending(h);  // the state is being overwritten
+    refresh_layout(h);
     if (h->ident) {
         check(hipMemcpyAsync(h->re, re, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy re");
         check(hipMemcpyAsync(h->im, im, h->len * sizeof(T), hipMemcpyHostToDevice, h->stream), "copy im");
@@ -895,9 +1007,22 @@ bool sorted_distinct(const uint32_t *qs, uint32_t k, uint32_t n) {
 
 }  // namespace
 
+namespace zeno {
+
+void backend_lookahead(zb_backend *h, const uint64_t *need, size_t count) {
+    h->la_need = need;
+    h->la_count = need ? count : 0;
+    h->la_pos = 0;
+}
+
+void backend_lookahead_pos(zb_backend *h, size_t pos) { h->la_pos = pos; }
+
+}  // namespace zeno
+
 extern "C" {
 
 const char *zb_last_error(void) { return last_error(); }
+
 
 namespace {
 
@@ -977,6 +1102,7 @@ int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
     else if (k == "defer_cx") h->defer_cx = value != 0;
     else if (k == "mfma") h->mfma = value != 0;
     else if (k == "fma") h->fma = value != 0;
+    else if (k == "relabel") h->relabel = value != 0;
     else if (k == "layout" && value >= -1 && value <= (1 << 30)) {
         h->sync();
         set_layout(h, (int)value);
@@ -1168,6 +1294,7 @@ int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
 int zb_chunk_sums(zb_backend *h, double *out) {
     return guarded([&] {
         flush(h);
+        refresh_layout(h);
         const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
         const uint64_t nch = (h->len + chunk - 1) / chunk;
         double *d = (double *)h->scratch_get(nch * 8);
@@ -1298,8 +1425,8 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
 }
 
 int zb_tune_default(uint32_t n_qubits, int precision, const char *key, int64_t value) {
-    static const char *keys[] = {"prefetch", "win_grid", "slots",  "page_bits", "tile_bits",
-                                 "defer_cx", "layout",   "mfma", "fma"};
+    static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits",
+                                 "defer_cx", "layout",   "mfma",  "fma",       "relabel"};
     const std::string k(key ? key : "");
     if (std::find(std::begin(keys), std::end(keys), k) == std::end(keys) || (precision != 32 && precision != 64)) {
         set_error("zb_tune_default: unknown key or precision: " + k);
@@ -1350,6 +1477,8 @@ int zb_max_abs_diff(zb_backend *a, zb_backend *b, double *out) {
     return guarded([&] {
         flush(a);
         flush(b);
+        refresh_layout(a);
+        refresh_layout(b);
         b->sync();
         unsigned long long *d = reinterpret_cast<unsigned long long *>(a->scratch_get(8));
         check(hipMemsetAsync(d, 0, 8, a->stream), "hipMemsetAsync");

@@ -97,6 +97,30 @@ void apply_op(zb_backend *h, const Op &o) {  // exec.rs:398-405
         ok(zb_apply_cx(h, o.a, o.b));
 }
 
+// Applies ops [lo, hi) in order, with the backend told which circuit
+// qubits each coming op needs in a tile (its window scheduling uses it).
+void apply_ops_ahead(zb_backend *h, const std::vector<Op> &ops, size_t lo, size_t hi) {
+    std::vector<uint64_t> need(hi > lo ? hi - lo : 0, 0);
+    for (size_t i = lo; i < hi; ++i) {
+        const Op &o = ops[i];
+        if (o.kind == kUnitary)
+            for (uint32_t q : o.qs) need[i - lo] |= 1ull << q;
+        else if (o.kind == kCx)
+            need[i - lo] = 1ull << o.b;
+    }
+    backend_lookahead(h, need.data(), need.size());
+    try {
+        for (size_t i = lo; i < hi; ++i) {
+            backend_lookahead_pos(h, i - lo);
+            apply_op(h, ops[i]);
+        }
+    } catch (...) {
+        backend_lookahead(h, nullptr, 0);
+        throw;
+    }
+    backend_lookahead(h, nullptr, 0);
+}
+
 // exec.rs:556-619
 void exec_shot_op(zb_backend *h, const Op &o, const zr_noise *nz, Rng &rng, uint64_t &clbits) {
     if (o.is_if) {
@@ -164,7 +188,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
         ok(zb_set_mode(h, mode));
         if (!state_is_zero) ok(zb_reset_all(h));
         if (!nz && !c.dynamic) {  // run_sampled (exec.rs:407-452)
-            for (const Op &o : c.ops) apply_op(h, o);
+            apply_ops_ahead(h, c.ops, 0, c.ops.size());
             ok(zb_finish(h));
             if (!c.final_measures.empty() && shots > 0) {
                 std::vector<uint64_t> smp(shots);
@@ -263,7 +287,8 @@ int64_t zr_compiled_final_measures(const zr_compiled *c, uint32_t *pairs) {
 int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi) {
     hi = std::min<int64_t>(hi, (int64_t)c->c.ops.size());
     try {
-        for (int64_t i = std::max<int64_t>(lo, 0); i < hi; ++i) apply_op(h, c->c.ops[(size_t)i]);
+        const int64_t from = std::max<int64_t>(lo, 0);
+        if (from < hi) apply_ops_ahead(h, c->c.ops, (size_t)from, (size_t)hi);
     } catch (const RunError &e) {
         return e.code;
     }

@@ -99,6 +99,13 @@ namespace zeno {
 int run_compiled_on(zb_backend *h, const zr_compiled *c, uint64_t shots, uint64_t seed, const zr_noise *noise,
                     int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out);
 
+// Window scheduling hint: need[i] is the mask of circuit qubits that the
+// i-th coming op needs inside a tile (a dense op's qubits, a CX's target),
+// valid until cleared with (nullptr, 0); pos is the op about to be applied.
+// With it, windows relabel qubits on write-back (results do not change).
+void backend_lookahead(zb_backend *h, const uint64_t *need, size_t count);
+void backend_lookahead_pos(zb_backend *h, size_t pos);
+
 // Thread-local last-error plumbing for the C ABI.
 void set_error(const std::string &msg);
 const char *last_error();

@@ -478,6 +478,12 @@ struct WinHeader {
     uint32_t m, nstages, nops, desc_words;
     uint32_t stage_off, op_off, data_off, nslots;  // word offsets inside the blob
     uint32_t q[kMaxTileBits];
+    // Relabelling of the tile's qubits on write-back (relabel != 0): the
+    // last stage scatters local bit i of an amplitude's index to LDS bit
+    // wb[i], so the tile leaves in the permuted layout.  wb is a product of
+    // disjoint swaps.
+    uint32_t relabel;
+    uint32_t wb[kMaxTileBits];
 };
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
@@ -640,6 +646,7 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
     for (int i = 0; i < LB; ++i)
         if ((t >> i) & 1) lo |= 1ull << hdr->q[i];
     const int lt = padded(t);  // LDS slot of local index t (+ (B + B/16)*j for row j)
+    const bool relabel = hdr->relabel != 0;
     auto tile_base = [&](uint64_t tile) {
         uint64_t b = tile;
 #pragma unroll
@@ -793,13 +800,25 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                         break;
                 }
             }
+            // the last stage of a relabelling window scatters in the new layout
+            const bool perm = relabel && s == nst - 1;
+            int slb = plb, ss[NS];
+#pragma unroll
+            for (int i = 0; i < NS; ++i) ss[i] = perm ? (int)hdr->wb[sl[i]] : sl[i];
+            if (perm) {
+                int v = 0;
+#pragma unroll
+                for (int i = 0; i < TB; ++i)
+                    if ((lb >> i) & 1) v |= 1 << hdr->wb[i];
+                slb = padded(v);
+            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 int c = 0;
 #pragma unroll
-                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
-                sre[plb + padded(c)] = xr[r];
-                sim[plb + padded(c)] = xi[r];
+                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << ss[i];
+                sre[slb + padded(c)] = xr[r];
+                sim[slb + padded(c)] = xi[r];
             }
             __syncthreads();
         }

@@ -38,6 +38,7 @@ SEARCH = {
     "prefetch": [1, 0],
     "win_grid": [16384, 4096, 65536],
     "defer_cx": [1, 0],
+    "relabel": [1, 0],
     "layout": [-1, 0],
 }
 FUSION = [1, 2, 3, 4]
