@@ -201,6 +201,27 @@ def test_relabelling_windows_bit_exact(orc, prec):
             assert np.array_equal(g.sample(5000, 3), o.sample(5000, 3))
 
 
+@pytest.mark.parametrize("prec", [32, 64])
+def test_relabelling_large_state_deterministic(prec):
+    """At 28 qubits (65536 tiles, every wave of a workgroup busy) the
+    permuted write-back scatter of a relabelling window reads positions that
+    other waves overwrite: the result must equal relabelling off exactly, on
+    every repeat."""
+    n = 28
+    comp = z.compile(z.matching_circuit(n, 5, 3, 5).to_program()[0], n, fusion=1)
+    with z.HipBackend(n, prec) as f:
+        f.tune("relabel", 0)
+        f.apply_compiled(comp)
+        ref = f.norm_sqr()
+        for rep in range(3):
+            _say(f"relabel n=28 prec={prec} repeat {rep}")
+            with z.HipBackend(n, prec) as g:
+                g.apply_compiled(comp)
+                assert list(g.layout()) != list(f.layout()), "no window relabelled"
+                assert g.max_abs_diff(f) == 0.0, rep
+                assert g.norm_sqr() == ref, rep
+
+
 def _unitary(rng, k):
     q, r = np.linalg.qr(rng.normal(size=(1 << k, 1 << k)) + 1j * rng.normal(size=(1 << k, 1 << k)))
     return q * (np.diag(r) / np.abs(np.diag(r)))

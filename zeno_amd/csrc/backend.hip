@@ -234,7 +234,7 @@ void set_layout(zb_backend *h, int kind) {
     upload_layout(h);
 }
 
-// Uploads the LUT of phys[] (index -> state index), synchronously.
+// Uploads the LUT of phys[] (index -> state index); returns when it landed.
 void upload_layout(zb_backend *h) {
     const uint32_t n = h->n;
     h->lut_dirty = false;
@@ -252,7 +252,11 @@ void upload_layout(zb_backend *h) {
             lut[(size_t)c * 256 + v] = o;
         }
     if (!h->lut_dev) check(hipMalloc((void **)&h->lut_dev, lut.size() * 8), "hipMalloc layout");
-    check(hipMemcpy(h->lut_dev, lut.data(), lut.size() * 8, hipMemcpyHostToDevice), "layout upload");
+    // On the backend's own stream and waited for: a plain hipMemcpy from
+    // pageable memory may return before its DMA lands, and the backend's
+    // non-blocking stream would not wait for it.
+    check(hipMemcpyAsync(h->lut_dev, lut.data(), lut.size() * 8, hipMemcpyHostToDevice, h->stream), "layout upload");
+    h->sync();
 }
 
 template <typename T>

@@ -802,6 +802,9 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
             }
             // the last stage of a relabelling window scatters in the new layout
             const bool perm = relabel && s == nst - 1;
+            // the permuted scatter writes other lanes' positions of this
+            // stage: every wave must have read its own before any writes
+            if (perm) __syncthreads();
             int slb = plb, ss[NS];
 #pragma unroll
             for (int i = 0; i < NS; ++i) ss[i] = perm ? (int)hdr->wb[sl[i]] : sl[i];
