@@ -101,10 +101,6 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
     hcheck(hipMalloc(&dmats.p, mats.size() * sizeof(T)), "hipMalloc mats");
     hcheck(hipMalloc(&dfm.p, fm.size() * 4), "hipMalloc fm");
     hcheck(hipMalloc(&dout.p, std::max<uint64_t>(shots, 1) * 8), "hipMalloc out");
-    hcheck(hipMemcpy(dops.p, ops.data(), ops.size() * sizeof(dev::TrajOp), hipMemcpyHostToDevice), "ops");
-    hcheck(hipMemcpy(dmats.p, mats.data(), mats.size() * sizeof(T), hipMemcpyHostToDevice), "mats");
-    hcheck(hipMemcpy(dfm.p, fm.data(), fm.size() * 4, hipMemcpyHostToDevice), "fm");
-
     dev::TrajNoise nz{};
     int noisy = 0;
     if (noise) {
@@ -118,6 +114,14 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
     hipGraphExec_t exec = nullptr;
     const auto t0 = std::chrono::steady_clock::now();
     try {
+        // Uploads on the launch stream, waited for: a plain hipMemcpy from
+        // pageable memory may return before its DMA lands, and a
+        // non-blocking stream would not wait for it.
+        hcheck(hipMemcpyAsync(dops.p, ops.data(), ops.size() * sizeof(dev::TrajOp), hipMemcpyHostToDevice, stream),
+               "ops");
+        hcheck(hipMemcpyAsync(dmats.p, mats.data(), mats.size() * sizeof(T), hipMemcpyHostToDevice, stream), "mats");
+        hcheck(hipMemcpyAsync(dfm.p, fm.data(), fm.size() * 4, hipMemcpyHostToDevice, stream), "fm");
+        hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
         // One persistent launch, captured in a graph and replayed once.
         hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipLaunchKernelGGL(dev::k_trajectories<T>, dim3((unsigned)grid), dim3(dev::kBlock), 0, stream, ws.get<T>(),

@@ -11,7 +11,8 @@
 // shot's clbit word — equals the CPU path's for the same seed.
 //
 // Every helper is force-inlined: the kernel then makes no device-function
-// calls, needs no scratch, and keeps 4 (f64) / 6 (f32) waves per SIMD.
+// calls and needs no scratch.  No-jump damping steps are folded into the
+// next sweep (Pend below), which removes about half of the state passes.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -84,6 +85,46 @@ struct TrajCtx {
     const T *pauli;  // LDS: X then Y as [re 4 | im 4]
 };
 
+// Pending damping scales.  A no-jump amplitude-damping step is the
+// diagonal diag(sc, sc*sqrt(1-gamma)) on one qubit (noise.rs:114-127).
+// Instead of a sweep of its own it is held here (uniform across the
+// workgroup) and applied to every amplitude by the next sweep as that sweep
+// loads it, with kern_diag1_scalar's expressions (imaginary parts T(0),
+// signed zeros included) and in the order the steps were taken; prob_one
+// applies it on the fly without writing.  The state therefore goes through
+// exactly the values of the one-sweep-per-step path in fewer passes.
+template <typename T>
+struct Pend {
+    int n = 0;
+    uint32_t q[kTrajMaxK];
+    T d0[kTrajMaxK], d1[kTrajMaxK];
+};
+
+template <typename T>
+__device__ __forceinline__ void pend_apply(const Pend<T> &p, uint64_t idx, T &ar, T &ai) {
+#pragma unroll
+    for (int e = 0; e < kTrajMaxK; ++e) {
+        if (e >= p.n) break;
+        const T d = ((idx >> p.q[e]) & 1) ? p.d1[e] : p.d0[e], z = T(0);
+        const T r = d * ar - z * ai, i = d * ai + z * ar;
+        ar = r;
+        ai = i;
+    }
+}
+
+// Applies the pending scales with a sweep of their own.
+template <typename T>
+__device__ __forceinline__ void t_pend_flush(const TrajCtx<T> c, Pend<T> &p) {
+    for (uint64_t i = threadIdx.x; i < c.len; i += blockDim.x) {
+        T ar = c.re[i], ai = c.im[i];
+        pend_apply(p, i, ar, ai);
+        c.re[i] = ar;
+        c.im[i] = ai;
+    }
+    p.n = 0;
+    __syncthreads();
+}
+
 // The state of a trajectory lives in L2 / the last-level cache; a sweep is
 // bound by the latency of those loads.  Each lane therefore loads a batch
 // of kTrajBatch<T> independent pairs before computing and storing any of
@@ -92,7 +133,7 @@ template <typename T>
 constexpr int kTrajBatch = sizeof(T) == 4 ? 8 : 4;
 
 template <typename T>
-__device__ __forceinline__ void t_1q(const TrajCtx<T> c, uint32_t q, const T *m) {
+__device__ __forceinline__ void t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t q, const T *m) {
     const T m00r = m[0], m01r = m[1], m10r = m[2], m11r = m[3];
     const T m00i = m[4], m01i = m[5], m10i = m[6], m11i = m[7];
     constexpr int U = kTrajBatch<T>;
@@ -112,6 +153,8 @@ __device__ __forceinline__ void t_1q(const TrajCtx<T> c, uint32_t q, const T *m)
             const uint64_t h = h0 + u * step;
             if (h < half) {
                 const uint64_t a = insert_zero(h, q), b = a + s;
+                pend_apply(p, a, ar[u], ai[u]);
+                pend_apply(p, b, br[u], bi[u]);
                 c.re[a] = m00r * ar[u] - m00i * ai[u] + m01r * br[u] - m01i * bi[u];
                 c.im[a] = m00r * ai[u] + m00i * ar[u] + m01r * bi[u] + m01i * br[u];
                 c.re[b] = m10r * ar[u] - m10i * ai[u] + m11r * br[u] - m11i * bi[u];
@@ -119,11 +162,12 @@ __device__ __forceinline__ void t_1q(const TrajCtx<T> c, uint32_t q, const T *m)
             }
         }
     }
+    p.n = 0;
     __syncthreads();
 }
 
 template <typename T, int K>
-__device__ __forceinline__ void t_kq(const TrajCtx<T> c, const uint32_t *qs, const T *m) {
+__device__ __forceinline__ void t_kq(const TrajCtx<T> c, Pend<T> &p, const uint32_t *qs, const T *m) {
     constexpr int D = 1 << K;
     for (uint64_t g = threadIdx.x; g < (c.len >> K); g += blockDim.x) {
         uint64_t base = g;
@@ -138,6 +182,7 @@ __device__ __forceinline__ void t_kq(const TrajCtx<T> c, const uint32_t *qs, con
                 if ((j >> b) & 1) o |= 1ull << qs[b];
             xr[j] = c.re[o];
             xi[j] = c.im[o];
+            pend_apply(p, o, xr[j], xi[j]);
         }
         // Rows stay a runtime loop so the LDS matrix is re-read per group
         // instead of being hoisted into registers.
@@ -158,12 +203,13 @@ __device__ __forceinline__ void t_kq(const TrajCtx<T> c, const uint32_t *qs, con
             c.im[o] = ai;
         }
     }
+    p.n = 0;
     __syncthreads();
 }
 
 // kern_diag1_scalar (src/state.rs:498-527): d0 on the a side, d1 on the b side.
 template <typename T>
-__device__ __forceinline__ void t_diag1(const TrajCtx<T> c, uint32_t q, T d0r, T d1r, T d0i, T d1i) {
+__device__ __forceinline__ void t_diag1(const TrajCtx<T> c, Pend<T> &p, uint32_t q, T d0r, T d1r, T d0i, T d1i) {
     constexpr int U = kTrajBatch<T>;
     const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
     for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
@@ -181,6 +227,8 @@ __device__ __forceinline__ void t_diag1(const TrajCtx<T> c, uint32_t q, T d0r, T
             const uint64_t h = h0 + u * step;
             if (h < half) {
                 const uint64_t a = insert_zero(h, q), b = a + s;
+                pend_apply(p, a, ar[u], ai[u]);
+                pend_apply(p, b, br[u], bi[u]);
                 c.re[a] = d0r * ar[u] - d0i * ai[u];
                 c.im[a] = d0r * ai[u] + d0i * ar[u];
                 c.re[b] = d1r * br[u] - d1i * bi[u];
@@ -188,15 +236,16 @@ __device__ __forceinline__ void t_diag1(const TrajCtx<T> c, uint32_t q, T d0r, T
             }
         }
     }
+    p.n = 0;
     __syncthreads();
 }
 
 // Diagonal op; d = [re D | im D] (k == 1: d0r, d1r, d0i, d1i).
 template <typename T>
-__device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, int k, const T *d) {
+__device__ __forceinline__ void t_diag(const TrajCtx<T> c, Pend<T> &p, const uint32_t *qs, int k, const T *d) {
     const int D = 1 << k;
     if (k == 1) {
-        t_diag1(c, qs[0], d[0], d[1], d[2], d[3]);
+        t_diag1(c, p, qs[0], d[0], d[1], d[2], d[3]);
         return;
     } else {
         constexpr int U = kTrajBatch<T>;
@@ -216,6 +265,7 @@ __device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, i
 #pragma unroll
                     for (int b = 0; b < kTrajMaxK; ++b)
                         if (b < k) j |= (uint32_t)((i >> qs[b]) & 1) << b;
+                    pend_apply(p, i, ar[u], ai[u]);
                     const T wr = d[j], wi = d[D + j];
                     c.re[i] = wr * ar[u] - wi * ai[u];
                     c.im[i] = wr * ai[u] + wi * ar[u];
@@ -223,44 +273,85 @@ __device__ __forceinline__ void t_diag(const TrajCtx<T> c, const uint32_t *qs, i
             }
         }
     }
+    p.n = 0;
     __syncthreads();
 }
 
+// apply_cx (src/state.rs:394-417).  With damping scales pending, every
+// amplitude needs them, so the sweep then covers both halves.
 template <typename T>
-__device__ __forceinline__ void t_cx(const TrajCtx<T> c, uint32_t ctl, uint32_t tgt) {
+__device__ __forceinline__ void t_cx(const TrajCtx<T> c, Pend<T> &p, uint32_t ctl, uint32_t tgt) {
     const uint32_t lo = ctl < tgt ? ctl : tgt, hi = ctl < tgt ? tgt : ctl;
     constexpr int U = kTrajBatch<T>;
     const uint64_t quarter = c.len >> 2, step = (uint64_t)blockDim.x;
-    for (uint64_t g0 = threadIdx.x; g0 < quarter; g0 += U * step) {
-        T r0[U], r1[U], i0[U], i1[U];
+    const uint64_t cb = 1ull << ctl, tb = 1ull << tgt;
+    if (p.n == 0) {
+        for (uint64_t g0 = threadIdx.x; g0 < quarter; g0 += U * step) {
+            T r0[U], r1[U], i0[U], i1[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t g = g0 + u * step;
-            if (g < quarter) {
-                const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
-                r0[u] = c.re[a], r1[u] = c.re[b], i0[u] = c.im[a], i1[u] = c.im[b];
+            for (int u = 0; u < U; ++u) {
+                const uint64_t g = g0 + u * step;
+                if (g < quarter) {
+                    const uint64_t a = insert_zero(insert_zero(g, lo), hi) | cb, b = a | tb;
+                    r0[u] = c.re[a], r1[u] = c.re[b], i0[u] = c.im[a], i1[u] = c.im[b];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const uint64_t g = g0 + u * step;
+                if (g < quarter) {
+                    const uint64_t a = insert_zero(insert_zero(g, lo), hi) | cb, b = a | tb;
+                    c.re[a] = r1[u];
+                    c.re[b] = r0[u];
+                    c.im[a] = i1[u];
+                    c.im[b] = i0[u];
+                }
             }
         }
+    } else {
+        constexpr int V = U / 2 > 0 ? U / 2 : 1;
+        for (uint64_t g0 = threadIdx.x; g0 < quarter; g0 += V * step) {
+            T xr[V][4], xi[V][4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const uint64_t g = g0 + u * step;
-            if (g < quarter) {
-                const uint64_t a = insert_zero(insert_zero(g, lo), hi) | (1ull << ctl), b = a | (1ull << tgt);
-                c.re[a] = r1[u];
-                c.re[b] = r0[u];
-                c.im[a] = i1[u];
-                c.im[b] = i0[u];
+            for (int u = 0; u < V; ++u) {
+                const uint64_t g = g0 + u * step;
+                if (g < quarter) {
+                    const uint64_t e = insert_zero(insert_zero(g, lo), hi);
+                    const uint64_t idx[4] = {e, e | tb, e | cb, e | cb | tb};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) xr[u][v] = c.re[idx[v]], xi[u][v] = c.im[idx[v]];
+                }
+            }
+#pragma unroll
+            for (int u = 0; u < V; ++u) {
+                const uint64_t g = g0 + u * step;
+                if (g < quarter) {
+                    const uint64_t e = insert_zero(insert_zero(g, lo), hi);
+                    const uint64_t idx[4] = {e, e | tb, e | cb, e | cb | tb};
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) pend_apply(p, idx[v], xr[u][v], xi[u][v]);
+                    c.re[idx[0]] = xr[u][0];
+                    c.im[idx[0]] = xi[u][0];
+                    c.re[idx[1]] = xr[u][1];
+                    c.im[idx[1]] = xi[u][1];
+                    c.re[idx[2]] = xr[u][3];  // the control = 1 pair swaps
+                    c.im[idx[2]] = xi[u][3];
+                    c.re[idx[3]] = xr[u][2];
+                    c.im[idx[3]] = xi[u][2];
+                }
             }
         }
+        p.n = 0;
     }
     __syncthreads();
 }
 
 // prob_one in the backend's fixed order: 4096-element blocks (16 serial
 // adds per lane, pairwise tree over 256 lanes), then the block sums the
-// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).
+// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).  Pending
+// damping scales are applied to each loaded amplitude, not stored.
 template <typename T>
-__device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, uint32_t q) {
+__device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, const Pend<T> &p, uint32_t q) {
     const uint64_t half = c.len >> 1, qs = 1ull << q;
     const int t = threadIdx.x;
     const int nb = (int)((half + kReduceSpan - 1) / kReduceSpan);
@@ -276,7 +367,9 @@ __device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, uint32_t q) {
                 const uint64_t e = base + (uint64_t)j * kBlock + t;
                 if (e < half) {
                     const uint64_t idx = insert_zero(e, q) + qs;
-                    const double r = (double)c.re[idx], m = (double)c.im[idx];
+                    T vr = c.re[idx], vi = c.im[idx];
+                    pend_apply(p, idx, vr, vi);
+                    const double r = (double)vr, m = (double)vi;
                     a = a + (r * r + m * m);
                 }
             }
@@ -310,7 +403,7 @@ __device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, uint32_t q) {
 }
 
 template <typename T>
-__device__ __forceinline__ void t_collapse(const TrajCtx<T> c, uint32_t q, int outcome, T scale) {
+__device__ __forceinline__ void t_collapse(const TrajCtx<T> c, Pend<T> &p, uint32_t q, int outcome, T scale) {
     constexpr int U = kTrajBatch<T>;
     const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
     for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
@@ -329,6 +422,7 @@ __device__ __forceinline__ void t_collapse(const TrajCtx<T> c, uint32_t q, int o
             if (h < half) {
                 const uint64_t a = insert_zero(h, q);
                 const uint64_t keep = outcome ? a + s : a, zero = outcome ? a : a + s;
+                pend_apply(p, keep, kr[u], ki[u]);
                 c.re[keep] = kr[u] * scale;
                 c.im[keep] = ki[u] * scale;
                 c.re[zero] = T(0);
@@ -336,52 +430,59 @@ __device__ __forceinline__ void t_collapse(const TrajCtx<T> c, uint32_t q, int o
             }
         }
     }
+    p.n = 0;
     __syncthreads();
 }
 
 // CpuBackend::measure (src/exec.rs:201-207)
 template <typename T>
-__device__ __forceinline__ int t_measure(const TrajCtx<T> c, uint32_t q, double u) {
-    const double p1 = t_prob_one(c, q);
+__device__ __forceinline__ int t_measure(const TrajCtx<T> c, Pend<T> &p, uint32_t q, double u) {
+    const double p1 = t_prob_one(c, p, q);
     const int outcome = u < p1;
-    const double p = outcome ? p1 : 1.0 - p1;
-    t_collapse(c, q, outcome, (T)(1.0 / sqrt(p)));
+    const double pr = outcome ? p1 : 1.0 - p1;
+    t_collapse(c, p, q, outcome, (T)(1.0 / sqrt(pr)));
     return outcome;
 }
 
 // noise.rs:91-100, through the same unitary / diagonal arithmetic the CPU
 // path uses (signed zeros included).
 template <typename T>
-__device__ __forceinline__ void t_pauli(const TrajCtx<T> c, uint32_t q, uint64_t code) {
-    if (code == 1) t_1q(c, q, c.pauli + 0);
-    else if (code == 2) t_1q(c, q, c.pauli + 8);
-    else if (code == 3) t_diag1(c, q, T(1), T(-1), T(0), T(0));
+__device__ __forceinline__ void t_pauli(const TrajCtx<T> c, Pend<T> &p, uint32_t q, uint64_t code) {
+    if (code == 1) t_1q(c, p, q, c.pauli + 0);
+    else if (code == 2) t_1q(c, p, q, c.pauli + 8);
+    else if (code == 3) t_diag1(c, p, q, T(1), T(-1), T(0), T(0));
 }
 
 template <typename T>
-__device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, const TrajNoise &nz, const uint32_t *qs, int k, DevRng &rng) {
+__device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, Pend<T> &p, const TrajNoise &nz, const uint32_t *qs,
+                                             int k, DevRng &rng) {
     const double pd = k == 1 ? nz.dep1 : nz.dep2;
     if (pd > 0.0 && rng.f64() < pd) {
         const uint64_t idx = rng.range(1, 1ull << (2 * k));
 #pragma unroll
         for (int b = 0; b < kTrajMaxK; ++b)
-            if (b < k) t_pauli(c, qs[b], (idx >> (2 * b)) & 3);
+            if (b < k) t_pauli(c, p, qs[b], (idx >> (2 * b)) & 3);
     }
 #pragma unroll
     for (int b = 0; b < kTrajMaxK; ++b) {
         if (b >= k) break;
         const uint32_t q = qs[b];
-        if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, q, 1);
-        if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, q, 3);
+        if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, p, q, 1);
+        if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, p, q, 3);
         if (nz.damping > 0.0) {  // noise.rs:114-127
             const double u = rng.f64();
-            const double pj = nz.damping * t_prob_one(c, q);
+            const double pj = nz.damping * t_prob_one(c, p, q);
             if (u < pj) {
-                t_measure(c, q, 0.0);
-                t_pauli(c, q, 1);
-            } else {
+                t_measure(c, p, q, 0.0);
+                t_pauli(c, p, q, 1);
+            } else {  // diag(sc, sc*sqrt(1-gamma)) on q, held for the next sweep
                 const double sc = 1.0 / sqrt(1.0 - pj);
-                t_diag1(c, q, (T)sc, (T)(sc * sqrt(1.0 - nz.damping)), T(0), T(0));
+                if (p.n == kTrajMaxK) t_pend_flush(c, p);  // not reached: a gate has k <= 3 qubits
+                const T d0 = (T)sc, d1 = (T)(sc * sqrt(1.0 - nz.damping));
+#pragma unroll
+                for (int e = 0; e < kTrajMaxK; ++e)  // static indices keep Pend in registers
+                    if (e == p.n) p.q[e] = q, p.d0[e] = d0, p.d1[e] = d1;
+                ++p.n;
             }
         }
     }
@@ -418,6 +519,7 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uin
         __syncthreads();
         DevRng rng;
         rng.seed(DevRng::mix(seed ^ shot));
+        Pend<T> pend;
         uint64_t clbits = 0;
         for (uint32_t oi = 0; oi < nops; ++oi) {
             const TrajOp &op = ops[oi];
@@ -434,36 +536,36 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uin
             }
             switch (kind) {
                 case 0:
-                    if (k == 1) t_1q(c, qs[0], smat);
-                    else if (k == 2) t_kq<T, 2>(c, qs, smat);
-                    else t_kq<T, 3>(c, qs, smat);
-                    if (noisy) t_gate_noise(c, nz, qs, (int)k, rng);
+                    if (k == 1) t_1q(c, pend, qs[0], smat);
+                    else if (k == 2) t_kq<T, 2>(c, pend, qs, smat);
+                    else t_kq<T, 3>(c, pend, qs, smat);
+                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng);
                     break;
                 case 1:
-                    t_diag(c, qs, (int)k, smat);
-                    if (noisy) t_gate_noise(c, nz, qs, (int)k, rng);
+                    t_diag(c, pend, qs, (int)k, smat);
+                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng);
                     break;
                 case 2: {
-                    t_cx(c, qs[0], qs[1]);
+                    t_cx(c, pend, qs[0], qs[1]);
                     if (noisy) {
                         const uint32_t sq[kTrajMaxK] = {qs[0] < qs[1] ? qs[0] : qs[1], qs[0] < qs[1] ? qs[1] : qs[0], 0};
-                        t_gate_noise(c, nz, sq, 2, rng);
+                        t_gate_noise(c, pend, nz, sq, 2, rng);
                     }
                     break;
                 }
                 case 3: {
-                    int bit = t_measure(c, qs[0], rng.f64());
+                    int bit = t_measure(c, pend, qs[0], rng.f64());
                     if (noisy) bit = t_readout<T>(nz, bit, rng);
                     clbits = (clbits & ~(1ull << op.clbit)) | ((uint64_t)bit << op.clbit);
                     break;
                 }
                 case 4:
-                    if (t_measure(c, qs[0], rng.f64())) t_pauli(c, qs[0], 1);
+                    if (t_measure(c, pend, qs[0], rng.f64())) t_pauli(c, pend, qs[0], 1);
                     break;
             }
         }
         for (uint32_t f = 0; f < nfm; ++f) {
-            int bit = t_measure(c, fm[2 * f], rng.f64());
+            int bit = t_measure(c, pend, fm[2 * f], rng.f64());
             if (noisy) bit = t_readout<T>(nz, bit, rng);
             clbits = (clbits & ~(1ull << fm[2 * f + 1])) | ((uint64_t)bit << fm[2 * f + 1]);
         }
