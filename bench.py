@@ -362,10 +362,11 @@ def main():
     if not args.no_extra:
         _log("batched trajectories ...")
         out["trajectories"] = bench_trajectories(args)
-        _log("serving ...")
-        out["serve"] = bench_serve(args)
-        first = next(iter(out["serve"]["rates"].values()))
-        out["p99_latency_ms_mixed_serve"] = first["served"]["p99_ms"]
+        if args.serve_requests > 0:
+            _log("serving ...")
+            out["serve"] = bench_serve(args)
+            first = next(iter(out["serve"]["rates"].values()))
+            out["p99_latency_ms_mixed_serve"] = first["served"]["p99_ms"]
     print(json.dumps(out), flush=True)
 
 

@@ -579,8 +579,12 @@ void flush_T(zb_backend *h) {
     uint32_t idx = 0;
     for (uint32_t q = 0; q < h->n; ++q)
         if ((mask >> q) & 1) hdr.q[idx] = q, local[q] = (int)idx++;
-    for (uint32_t i = 0; i < m; ++i) hdr.wb[i] = i;
-    for (const auto &sw : h->wb_swaps) std::swap(hdr.wb[local[sw.first]], hdr.wb[local[sw.second]]);
+    // fwd[i]: the local bit that local bit i moves to; the kernel reads
+    // through the inverse
+    uint32_t fwd[dev::kMaxTileBits];
+    for (uint32_t i = 0; i < m; ++i) fwd[i] = i;
+    for (const auto &sw : h->wb_swaps) std::swap(fwd[local[sw.first]], fwd[local[sw.second]]);
+    for (uint32_t i = 0; i < m; ++i) hdr.wb[fwd[i]] = i;
     hdr.relabel = h->wb_swaps.empty() ? 0 : 1;
     h->wb_swaps.clear();
     // Stages: greedy over the ops in order; a stage's slots must hold every
@@ -721,9 +725,13 @@ void close_window(zb_backend *h) {
             for (; hit; hit &= hit - 1) next.emplace_back(i, (uint32_t)__builtin_ctzll(hit));
         }
         for (uint64_t r = open; r; r &= r - 1) next.emplace_back(SIZE_MAX, (uint32_t)__builtin_ctzll(r));
-        // soonest first; on ties the present low occupants stay
+        // soonest first; among qubits no coming op uses, circuit qubits
+        // 0..low-1 first (circuit-order reads of the finished state then
+        // walk whole 128-byte runs); on other ties the present low
+        // occupants stay
         std::stable_sort(next.begin(), next.end(), [&](const auto &a, const auto &b) {
             if (a.first != b.first) return a.first < b.first;
+            if (a.first == SIZE_MAX && (a.second < low) != (b.second < low)) return a.second < low;
             return (h->phys[a.second] < low) > (h->phys[b.second] < low);
         });
         uint64_t keep = 0;
@@ -1020,6 +1028,18 @@ void backend_lookahead(zb_backend *h, const uint64_t *need, size_t count) {
 }
 
 void backend_lookahead_pos(zb_backend *h, size_t pos) { h->la_pos = pos; }
+
+// End of a run with lookahead: the held-back CX ops are queued and the last
+// window closes as one that no coming op uses, which moves circuit qubits
+// 0..low-1 of its tile back onto the low state bits.
+int backend_end_run(zb_backend *h) {
+    return guarded([&] {
+        h->la_pos = h->la_count;
+        if (h->prec == 64) settle<double>(h, ~0ull);
+        else settle<float>(h, ~0ull);
+        close_window(h);
+    });
+}
 
 }  // namespace zeno
 

@@ -114,6 +114,7 @@ void apply_ops_ahead(zb_backend *h, const std::vector<Op> &ops, size_t lo, size_
             backend_lookahead_pos(h, i - lo);
             apply_op(h, ops[i]);
         }
+        ok(backend_end_run(h));
     } catch (...) {
         backend_lookahead(h, nullptr, 0);
         throw;

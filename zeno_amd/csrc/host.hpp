@@ -105,6 +105,10 @@ int run_compiled_on(zb_backend *h, const zr_compiled *c, uint64_t shots, uint64_
 // With it, windows relabel qubits on write-back (results do not change).
 void backend_lookahead(zb_backend *h, const uint64_t *need, size_t count);
 void backend_lookahead_pos(zb_backend *h, size_t pos);
+// Flushes the run's queued work, the last window homing circuit qubits
+// 0..low-1 onto the low state bits; call while the hint is still set.
+// Returns a ZB_* code.
+int backend_end_run(zb_backend *h);
 
 // Thread-local last-error plumbing for the C ABI.
 void set_error(const std::string &msg);

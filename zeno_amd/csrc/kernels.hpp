@@ -479,9 +479,9 @@ struct WinHeader {
     uint32_t stage_off, op_off, data_off, nslots;  // word offsets inside the blob
     uint32_t q[kMaxTileBits];
     // Relabelling of the tile's qubits on write-back (relabel != 0): the
-    // last stage scatters local bit i of an amplitude's index to LDS bit
-    // wb[i], so the tile leaves in the permuted layout.  wb is a product of
-    // disjoint swaps.
+    // amplitude written back to local index u is read from the LDS position
+    // whose bit wb[i] is bit i of u, so the tile leaves in the permuted
+    // layout.  wb is a bit permutation (the inverse of the relabelling's).
     uint32_t relabel;
     uint32_t wb[kMaxTileBits];
 };
@@ -646,7 +646,29 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
     for (int i = 0; i < LB; ++i)
         if ((t >> i) & 1) lo |= 1ull << hdr->q[i];
     const int lt = padded(t);  // LDS slot of local index t (+ (B + B/16)*j for row j)
-    const bool relabel = hdr->relabel != 0;
+    // Write-back read positions of local index t + B*j: wlt + woff[j].  A
+    // relabelling window reads through its bit map (bit i of the index
+    // selects LDS bit wb[i]); the map is a bit permutation, so the lane part
+    // and the row part stay disjoint and padded() splits over them.  woff
+    // is uniform: computed once per launch.
+    int wlt = lt, woff[R];
+#pragma unroll
+    for (int j = 0; j < R; ++j) woff[j] = (B + B / 16) * j;
+    if (hdr->relabel != 0) {
+        int v = 0;
+#pragma unroll
+        for (int i = 0; i < LB; ++i)
+            if ((t >> i) & 1) v |= 1 << hdr->wb[i];
+        wlt = padded(v);
+#pragma unroll
+        for (int j = 0; j < R; ++j) {
+            int u = 0;
+#pragma unroll
+            for (int i = LB; i < TB; ++i)
+                if ((j >> (i - LB)) & 1) u |= 1 << hdr->wb[i];
+            woff[j] = padded(u);
+        }
+    }
     auto tile_base = [&](uint64_t tile) {
         uint64_t b = tile;
 #pragma unroll
@@ -800,37 +822,25 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                         break;
                 }
             }
-            // the last stage of a relabelling window scatters in the new layout
-            const bool perm = relabel && s == nst - 1;
-            // the permuted scatter writes other lanes' positions of this
-            // stage: every wave must have read its own before any writes
-            if (perm) __syncthreads();
-            int slb = plb, ss[NS];
-#pragma unroll
-            for (int i = 0; i < NS; ++i) ss[i] = perm ? (int)hdr->wb[sl[i]] : sl[i];
-            if (perm) {
-                int v = 0;
-#pragma unroll
-                for (int i = 0; i < TB; ++i)
-                    if ((lb >> i) & 1) v |= 1 << hdr->wb[i];
-                slb = padded(v);
-            }
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 int c = 0;
 #pragma unroll
-                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << ss[i];
-                sre[slb + padded(c)] = xr[r];
-                sim[slb + padded(c)] = xi[r];
+                for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
+                sre[plb + padded(c)] = xr[r];
+                sim[plb + padded(c)] = xi[r];
             }
             __syncthreads();
         }
         // write-back through its own registers: pr/pi may already hold the
-        // prefetched next tile
+        // prefetched next tile.  The opaque copy of wlt makes the R row
+        // addresses per-tile adds instead of loop-invariant registers.
+        int w = wlt;
+        asm volatile("" : "+v"(w));
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            wr[j] = sre[lt + (B + B / 16) * j];
-            wi[j] = sim[lt + (B + B / 16) * j];
+            wr[j] = sre[w + woff[j]];
+            wi[j] = sim[w + woff[j]];
         }
         wbase = base;
         pending = true;
