@@ -894,6 +894,39 @@ __global__ __launch_bounds__(kBlock) void k_max_abs_diff(const A *__restrict__ a
 }
 
 // --------------------------------------------------------------- sampler
+// Lane 0's strictly ordered walk over a slab of squared moduli in LDS, in
+// register groups: two groups alternate, the loads of one issued before the
+// other's adds, so the serial f64 chain, not the LDS latency, sets the
+// pace.  group(g, v) takes the 16 values from index g and returns false to
+// stop; one(j, v) takes the tail one value at a time.
+constexpr int kSerialGroup = 16;
+
+__device__ __forceinline__ void load_group(double *v, const double *tv) {
+#pragma unroll
+    for (int u = 0; u < kSerialGroup; ++u) v[u] = tv[u];
+}
+
+template <typename Group, typename One>
+__device__ __forceinline__ void serial_walk(const double *tv, int cnt, Group &&group, One &&one) {
+    const int full = cnt / kSerialGroup * kSerialGroup;
+    double a[kSerialGroup], b[kSerialGroup];
+    if (full > 0) load_group(a, tv);
+    for (int g = 0; g < full;) {
+        const bool more_b = g + kSerialGroup < full;
+        if (more_b) load_group(b, tv + g + kSerialGroup);
+        if (!group(g, a)) return;
+        g += kSerialGroup;
+        if (!more_b) break;
+        const bool more_a = g + kSerialGroup < full;
+        if (more_a) load_group(a, tv + g + kSerialGroup);
+        if (!group(g, b)) return;
+        g += kSerialGroup;
+        if (!more_a) break;
+    }
+    for (int j = full; j < cnt; ++j)
+        if (!one(j, tv[j])) return;
+}
+
 // sample_indices pass 1 (src/sample.rs:38-50): per chunk a strictly serial
 // f64 sum acc += r*r + i*i in index order.  The 256 lanes load a 4096
 // element slab coalesced and square it into LDS; lane 0 then accumulates
@@ -914,7 +947,17 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re,
         }
         __syncthreads();
         if (threadIdx.x == 0)
-            for (int j = 0; j < cnt; ++j) acc += tv[j];
+            serial_walk(
+                tv, cnt,
+                [&](int, const double *v) {
+#pragma unroll
+                    for (int u = 0; u < kSerialGroup; ++u) acc += v[u];
+                    return true;
+                },
+                [&](int, double v) {
+                    acc += v;
+                    return true;
+                });
         __syncthreads();
     }
     if (threadIdx.x == 0) sums[blockIdx.x] = acc;
@@ -953,12 +996,37 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
         }
         __syncthreads();
         if (threadIdx.x == 0) {
+            // The next draw's threshold stays in a register, reloaded only
+            // when a draw is emitted (+inf once all are).  acc never
+            // decreases (it adds r*r + i*i >= 0 and rounding is monotonic),
+            // so a group whose last running sum does not pass the threshold
+            // emits nothing and needs no per-element test.
             uint64_t ptr = sptr;
-            for (int j = 0; j < cnt; ++j) {
-                acc += tv[j];
-                while (ptr < s1 && us[ptr] < acc) out[ptr++] = slab + (uint64_t)j;
-                if (ptr == s1) break;
-            }
+            double nu = ptr < s1 ? us[ptr] : __builtin_inf();
+            auto emit = [&](int j, double x) {
+                while (nu < x) {
+                    out[ptr++] = slab + (uint64_t)j;
+                    nu = ptr < s1 ? us[ptr] : __builtin_inf();
+                }
+            };
+            serial_walk(
+                tv, cnt,
+                [&](int g, const double *v) {
+                    double run[kSerialGroup], x = acc;
+#pragma unroll
+                    for (int u = 0; u < kSerialGroup; ++u) run[u] = x = x + v[u];
+                    acc = x;
+                    if (nu < x) {
+#pragma unroll
+                        for (int u = 0; u < kSerialGroup; ++u) emit(g + u, run[u]);
+                    }
+                    return ptr < s1;
+                },
+                [&](int j, double v) {
+                    acc += v;
+                    emit(j, acc);
+                    return ptr < s1;
+                });
             sptr = ptr;
             sdone = ptr == s1;
         }
