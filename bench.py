@@ -257,6 +257,11 @@ def main():
     err = be32.max_abs_diff(be64)
     norm64 = be64.norm_sqr()
     be32.close()
+    # sample_indices on the finished f64 state: 1000 shots, once warm
+    be64.sample(1000, args.seed)
+    t0 = time.perf_counter()
+    be64.sample(1000, args.seed + 1)
+    sample_ms = (time.perf_counter() - t0) * 1e3
 
     # Opt-in knobs that trade bit-identity with the CPU path for speed,
     # each timed once and checked against the exact f64 state above.
@@ -343,6 +348,7 @@ def main():
         "f32_speedup_vs_f64": round(s64 / s32, 3),
         "f32_max_abs_damp_vs_f64": err,
         "f64_norm_sqr": norm64,
+        "sample_1000_shots_ms_f64": round(sample_ms, 3),
         "kernels_f64": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls64.items()},
         "kernels_f32": {k: {"launches": v["launches"], "avg_us": round(1e3 * v["ms"] / v["launches"], 2)} for k, v in cls32.items()},
         "alternatives": alternatives,
