@@ -335,6 +335,27 @@ def test_sampler_indices_and_chunk_sums_bit_exact(orc):
             assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed))
 
 
+def test_sampler_concentrated_distributions_bit_exact(orc):
+    """Mass on a few indices: GHZ (index 0 and the last index, every chunk
+    between empty) and H on qubits 0..2 only (8 indices inside the first
+    16-value group), so runs of draws land on one index and most chunks
+    contribute nothing; the draws equal the oracle's."""
+    for n, kind in ((18, "ghz"), (21, "ghz"), (19, "h3")):
+        if kind == "ghz":
+            rows = [orc.instr("h", [0])] + [orc.instr("cx", [q, q + 1]) for q in range(n - 1)]
+        else:
+            rows = [orc.instr("h", [q]) for q in range(3)]
+        prog = orc.program(rows)
+        for prec in (64, 32):
+            _say(f"sample concentrated n={n} {kind} prec={prec}")
+            o = orc.State(n, prec)
+            o.apply_compiled(orc.Compiled(prog, n, fusion=1))
+            with z.HipBackend(n, prec) as g:
+                g.apply_compiled(z.compile(prog, n, fusion=1))
+                for shots, seed in ((1000, 11), (50000, 12)):
+                    assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed)), (n, kind, prec, shots)
+
+
 def test_measure_and_collapse_bit_exact(orc):
     n = 12
     for prec in (64, 32):
