@@ -125,6 +125,81 @@ __device__ __forceinline__ void t_pend_flush(const TrajCtx<T> c, Pend<T> &p) {
     __syncthreads();
 }
 
+// prob_one in the backend's fixed order: 4096-element blocks (16 serial
+// adds per lane, pairwise tree over 256 lanes), then the block sums the
+// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).  Pending
+// damping scales are applied to each loaded amplitude, not stored.
+//
+// t_prob_blocks: the lane partials of blocks b0 .. b0+nchunk-1 (nchunk <= 8)
+// are in red[bi * 256 + lane]; each block's pairwise tree goes to bsum.
+template <typename T>
+__device__ __forceinline__ void t_prob_blocks(const TrajCtx<T> c, int b0, int nchunk) {
+    double *s = c.red;
+    double *bsum = c.red + 8 * kBlock;
+    const int t = threadIdx.x;
+    __syncthreads();
+    for (int st = kBlock / 2; st >= 1; st >>= 1) {
+        if (t < st)
+            for (int bi = 0; bi < nchunk; ++bi) s[bi * kBlock + t] = s[bi * kBlock + t] + s[bi * kBlock + t + st];
+        __syncthreads();
+    }
+    if (t < nchunk) bsum[b0 + t] = s[t * kBlock];
+    __syncthreads();
+}
+
+// The block sums of all nb blocks, combined by the same tree.
+template <typename T>
+__device__ __forceinline__ double t_prob_final(const TrajCtx<T> c, int nb) {
+    double *s = c.red;
+    double *bsum = c.red + 8 * kBlock;
+    const int t = threadIdx.x;
+    if (nb > 1) {
+        double a = 0.0;
+        if (t < nb) a = a + bsum[t];  // the 4096-wide pass: only j = 0 is populated
+        s[t] = a;
+        __syncthreads();
+        for (int st = kBlock / 2; st >= 1; st >>= 1) {
+            if (t < st) s[t] = s[t] + s[t + st];
+            __syncthreads();
+        }
+        const double r = s[0];
+        __syncthreads();
+        return r;
+    }
+    const double r = bsum[0];
+    __syncthreads();
+    return r;
+}
+
+template <typename T>
+__device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, const Pend<T> &p, uint32_t q) {
+    const uint64_t half = c.len >> 1, qs = 1ull << q;
+    const int t = threadIdx.x;
+    const int nb = (int)((half + kReduceSpan - 1) / kReduceSpan);
+    double *s = c.red;  // [8][256] lane partials
+    for (int b0 = 0; b0 < nb; b0 += 8) {
+        const int nchunk = nb - b0 < 8 ? nb - b0 : 8;
+        for (int bi = 0; bi < nchunk; ++bi) {
+            double a = 0.0;
+            const uint64_t base = (uint64_t)(b0 + bi) * kReduceSpan;
+#pragma unroll
+            for (int j = 0; j < kReduceSpan / kBlock; ++j) {
+                const uint64_t e = base + (uint64_t)j * kBlock + t;
+                if (e < half) {
+                    const uint64_t idx = insert_zero(e, q) + qs;
+                    T vr = c.re[idx], vi = c.im[idx];
+                    pend_apply(p, idx, vr, vi);
+                    const double r = (double)vr, m = (double)vi;
+                    a = a + (r * r + m * m);
+                }
+            }
+            s[bi * kBlock + t] = a;
+        }
+        t_prob_blocks(c, b0, nchunk);
+    }
+    return t_prob_final(c, nb);
+}
+
 // The state of a trajectory lives in L2 / the last-level cache; a sweep is
 // bound by the latency of those loads.  Each lane therefore loads a batch
 // of kTrajBatch<T> independent pairs before computing and storing any of
@@ -132,12 +207,21 @@ __device__ __forceinline__ void t_pend_flush(const TrajCtx<T> c, Pend<T> &p) {
 template <typename T>
 constexpr int kTrajBatch = sizeof(T) == 4 ? 8 : 4;
 
+// With fuse (13 <= n <= 16: the b-side half is 1..8 whole 4096-element
+// blocks), the sweep also forms t_prob_one(q) of the new state: lane t visits
+// half indices t + 256 m in increasing m, which is exactly the lane's serial
+// order inside each block there, so it adds the same terms in the same
+// order into the same partials and finishes with the same tree.  Returns
+// that probability, or -1 without fuse.
 template <typename T>
-__device__ __forceinline__ void t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t q, const T *m) {
+__device__ __forceinline__ double t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t q, const T *m, bool fuse = false) {
     const T m00r = m[0], m01r = m[1], m10r = m[2], m11r = m[3];
     const T m00i = m[4], m01i = m[5], m10i = m[6], m11i = m[7];
     constexpr int U = kTrajBatch<T>;
+    static_assert(kReduceSpan / kBlock % U == 0, "a batch stays inside one reduction block");
     const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
+    double acc = 0.0;
+    int held = 0, blk = 0;
     for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
         T ar[U], ai[U], br[U], bi[U];
 #pragma unroll
@@ -157,13 +241,32 @@ __device__ __forceinline__ void t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t q,
                 pend_apply(p, b, br[u], bi[u]);
                 c.re[a] = m00r * ar[u] - m00i * ai[u] + m01r * br[u] - m01i * bi[u];
                 c.im[a] = m00r * ai[u] + m00i * ar[u] + m01r * bi[u] + m01i * br[u];
-                c.re[b] = m10r * ar[u] - m10i * ai[u] + m11r * br[u] - m11i * bi[u];
-                c.im[b] = m10r * ai[u] + m10i * ar[u] + m11r * bi[u] + m11i * br[u];
+                const T nr = m10r * ar[u] - m10i * ai[u] + m11r * br[u] - m11i * bi[u];
+                const T ni = m10r * ai[u] + m10i * ar[u] + m11r * bi[u] + m11i * br[u];
+                c.re[b] = nr;
+                c.im[b] = ni;
+                if (fuse) {
+                    const double r = (double)nr, mi = (double)ni;
+                    acc = acc + (r * r + mi * mi);
+                }
+            }
+        }
+        if (fuse) {
+            held += U;
+            if (held == kReduceSpan / kBlock) {
+                c.red[blk * kBlock + threadIdx.x] = acc;
+                acc = 0.0;
+                held = 0;
+                ++blk;
             }
         }
     }
     p.n = 0;
     __syncthreads();
+    if (!fuse) return -1.0;
+    const int nb = (int)(half / kReduceSpan);
+    t_prob_blocks(c, 0, nb);
+    return t_prob_final(c, nb);
 }
 
 template <typename T, int K>
@@ -346,62 +449,6 @@ __device__ __forceinline__ void t_cx(const TrajCtx<T> c, Pend<T> &p, uint32_t ct
     __syncthreads();
 }
 
-// prob_one in the backend's fixed order: 4096-element blocks (16 serial
-// adds per lane, pairwise tree over 256 lanes), then the block sums the
-// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).  Pending
-// damping scales are applied to each loaded amplitude, not stored.
-template <typename T>
-__device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, const Pend<T> &p, uint32_t q) {
-    const uint64_t half = c.len >> 1, qs = 1ull << q;
-    const int t = threadIdx.x;
-    const int nb = (int)((half + kReduceSpan - 1) / kReduceSpan);
-    double *s = c.red;                 // [8][256] lane partials
-    double *bsum = c.red + 8 * kBlock;  // block sums (<= 256)
-    for (int b0 = 0; b0 < nb; b0 += 8) {
-        const int nchunk = nb - b0 < 8 ? nb - b0 : 8;
-        for (int bi = 0; bi < nchunk; ++bi) {
-            double a = 0.0;
-            const uint64_t base = (uint64_t)(b0 + bi) * kReduceSpan;
-#pragma unroll
-            for (int j = 0; j < kReduceSpan / kBlock; ++j) {
-                const uint64_t e = base + (uint64_t)j * kBlock + t;
-                if (e < half) {
-                    const uint64_t idx = insert_zero(e, q) + qs;
-                    T vr = c.re[idx], vi = c.im[idx];
-                    pend_apply(p, idx, vr, vi);
-                    const double r = (double)vr, m = (double)vi;
-                    a = a + (r * r + m * m);
-                }
-            }
-            s[bi * kBlock + t] = a;
-        }
-        __syncthreads();
-        for (int st = kBlock / 2; st >= 1; st >>= 1) {
-            if (t < st)
-                for (int bi = 0; bi < nchunk; ++bi) s[bi * kBlock + t] = s[bi * kBlock + t] + s[bi * kBlock + t + st];
-            __syncthreads();
-        }
-        if (t < nchunk) bsum[b0 + t] = s[t * kBlock];
-        __syncthreads();
-    }
-    if (nb > 1) {
-        double a = 0.0;
-        if (t < nb) a = a + bsum[t];  // the 4096-wide pass: only j = 0 is populated
-        s[t] = a;
-        __syncthreads();
-        for (int st = kBlock / 2; st >= 1; st >>= 1) {
-            if (t < st) s[t] = s[t] + s[t + st];
-            __syncthreads();
-        }
-        const double r = s[0];
-        __syncthreads();
-        return r;
-    }
-    const double r = bsum[0];
-    __syncthreads();
-    return r;
-}
-
 template <typename T>
 __device__ __forceinline__ void t_collapse(const TrajCtx<T> c, Pend<T> &p, uint32_t q, int outcome, T scale) {
     constexpr int U = kTrajBatch<T>;
@@ -453,25 +500,28 @@ __device__ __forceinline__ void t_pauli(const TrajCtx<T> c, Pend<T> &p, uint32_t
     else if (code == 3) t_diag1(c, p, q, T(1), T(-1), T(0), T(0));
 }
 
+// p1_first: prob_one of qs[0] formed by the gate's own sweep (or < 0); it
+// stands until any Pauli changes the state.
 template <typename T>
 __device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, Pend<T> &p, const TrajNoise &nz, const uint32_t *qs,
-                                             int k, DevRng &rng) {
+                                             int k, DevRng &rng, double p1_first = -1.0) {
     const double pd = k == 1 ? nz.dep1 : nz.dep2;
     if (pd > 0.0 && rng.f64() < pd) {
         const uint64_t idx = rng.range(1, 1ull << (2 * k));
 #pragma unroll
         for (int b = 0; b < kTrajMaxK; ++b)
             if (b < k) t_pauli(c, p, qs[b], (idx >> (2 * b)) & 3);
+        p1_first = -1.0;
     }
 #pragma unroll
     for (int b = 0; b < kTrajMaxK; ++b) {
         if (b >= k) break;
         const uint32_t q = qs[b];
-        if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, p, q, 1);
-        if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, p, q, 3);
+        if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, p, q, 1), p1_first = -1.0;
+        if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, p, q, 3), p1_first = -1.0;
         if (nz.damping > 0.0) {  // noise.rs:114-127
             const double u = rng.f64();
-            const double pj = nz.damping * t_prob_one(c, p, q);
+            const double pj = nz.damping * (b == 0 && p1_first >= 0.0 ? p1_first : t_prob_one(c, p, q));
             if (u < pj) {
                 t_measure(c, p, q, 0.0);
                 t_pauli(c, p, q, 1);
@@ -510,6 +560,8 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uin
     }
     __syncthreads();
     const uint64_t len = 1ull << n;
+    // a 1q gate's sweep also forms the damping probability of its qubit
+    const bool fuse_p1 = noisy && nz.damping > 0.0 && n >= 13 && n <= 16;
     TrajCtx<T> c{ws + (uint64_t)blockIdx.x * 2 * len, ws + (uint64_t)blockIdx.x * 2 * len + len, len, red, spauli};
     for (uint64_t shot = blockIdx.x; shot < shots; shot += gridDim.x) {
         for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) {
@@ -535,12 +587,14 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uin
                 __syncthreads();
             }
             switch (kind) {
-                case 0:
-                    if (k == 1) t_1q(c, pend, qs[0], smat);
+                case 0: {
+                    double p1 = -1.0;
+                    if (k == 1) p1 = t_1q(c, pend, qs[0], smat, fuse_p1);
                     else if (k == 2) t_kq<T, 2>(c, pend, qs, smat);
                     else t_kq<T, 3>(c, pend, qs, smat);
-                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng);
+                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng, p1);
                     break;
+                }
                 case 1:
                     t_diag(c, pend, qs, (int)k, smat);
                     if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng);
