@@ -449,20 +449,21 @@ def test_batched_trajectories_match_oracle(orc, prec):
 
 @pytest.mark.parametrize("prec", [32, 64])
 def test_batched_trajectories_fused_damping_probability(orc, prec):
-    """At 13..16 qubits a 1q gate's sweep also forms its qubit's damping
-    probability; with every channel at high rates (Paulis often land
-    between the gate and its damping draw, which falls back to the separate
-    sum) each shot still equals the CPU trajectory sampler's."""
+    """From 13 qubits a 1q gate's sweep also forms its qubit's damping
+    probability (17 qubits: more than one group of 8 reduction blocks); with
+    every channel at high rates (Paulis often land between the gate and its
+    damping draw, which falls back to the separate sum) each shot still
+    equals the CPU trajectory sampler's."""
     nz = {"depolarizing_1q": 0.05, "depolarizing_2q": 0.05, "bit_flip": 0.05, "phase_flip": 0.05,
           "amplitude_damping": 0.1, "readout_flip_0to1": 0.01, "readout_flip_1to0": 0.01}
-    for n in (13, 14):
+    for n, shots in ((13, 40), (14, 40), (17, 16)):
         prog = orc.program([orc.gen_random(n, 6, 31 + n), orc.measure_all(n)])
         c = orc.Compiled(prog, n, fusion=0)
-        want, _ = orc.run(c, 40, 77, prec=prec, noise=nz)
+        want, _ = orc.run(c, shots, 77, prec=prec, noise=nz)
         _say(f"fused damping probability {n}q prec={prec}")
-        words, _ = z.run_trajectories(z.compile(prog, n, fusion=0), 40, 77, prec, z.NoiseModel(**nz))
+        words, _ = z.run_trajectories(z.compile(prog, n, fusion=0), shots, 77, prec, z.NoiseModel(**nz))
         assert _traj_counts(words) == want
-        for shot in range(0, 40, 9):
+        for shot in range(0, shots, 9):
             one, _ = orc.run(c, 1, 77 ^ shot, prec=prec, noise=nz)
             assert list(one) == [int(words[shot])], shot
 

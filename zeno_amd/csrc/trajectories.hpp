@@ -207,12 +207,13 @@ __device__ __forceinline__ double t_prob_one(const TrajCtx<T> c, const Pend<T> &
 template <typename T>
 constexpr int kTrajBatch = sizeof(T) == 4 ? 8 : 4;
 
-// With fuse (13 <= n <= 16: the b-side half is 1..8 whole 4096-element
-// blocks), the sweep also forms t_prob_one(q) of the new state: lane t visits
-// half indices t + 256 m in increasing m, which is exactly the lane's serial
-// order inside each block there, so it adds the same terms in the same
-// order into the same partials and finishes with the same tree.  Returns
-// that probability, or -1 without fuse.
+// With fuse (13 <= n <= 20: the b-side half is whole 4096-element blocks),
+// the sweep also forms t_prob_one(q) of the new state: lane t visits half
+// indices t + 256 m in increasing m, which is exactly the lane's serial order
+// inside each block there, so it adds the same terms in the same order into
+// the same partials; every lane reaches each group of 8 blocks at the same
+// iteration, where the group's trees run as in t_prob_one.  Returns that
+// probability, or -1 without fuse.
 template <typename T>
 __device__ __forceinline__ double t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t q, const T *m, bool fuse = false) {
     const T m00r = m[0], m01r = m[1], m10r = m[2], m11r = m[3];
@@ -221,7 +222,7 @@ __device__ __forceinline__ double t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t 
     static_assert(kReduceSpan / kBlock % U == 0, "a batch stays inside one reduction block");
     const uint64_t s = 1ull << q, half = c.len >> 1, step = (uint64_t)blockDim.x;
     double acc = 0.0;
-    int held = 0, blk = 0;
+    int held = 0, blk = 0, b0 = 0;
     for (uint64_t h0 = threadIdx.x; h0 < half; h0 += U * step) {
         T ar[U], ai[U], br[U], bi[U];
 #pragma unroll
@@ -257,16 +258,19 @@ __device__ __forceinline__ double t_1q(const TrajCtx<T> c, Pend<T> &p, uint32_t 
                 c.red[blk * kBlock + threadIdx.x] = acc;
                 acc = 0.0;
                 held = 0;
-                ++blk;
+                if (++blk == 8) {
+                    t_prob_blocks(c, b0, 8);
+                    b0 += 8;
+                    blk = 0;
+                }
             }
         }
     }
     p.n = 0;
     __syncthreads();
     if (!fuse) return -1.0;
-    const int nb = (int)(half / kReduceSpan);
-    t_prob_blocks(c, 0, nb);
-    return t_prob_final(c, nb);
+    if (blk > 0) t_prob_blocks(c, b0, blk);
+    return t_prob_final(c, (int)(half / kReduceSpan));
 }
 
 template <typename T, int K>
@@ -561,7 +565,7 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uin
     __syncthreads();
     const uint64_t len = 1ull << n;
     // a 1q gate's sweep also forms the damping probability of its qubit
-    const bool fuse_p1 = noisy && nz.damping > 0.0 && n >= 13 && n <= 16;
+    const bool fuse_p1 = noisy && nz.damping > 0.0 && n >= 13 && n <= 20;
     TrajCtx<T> c{ws + (uint64_t)blockIdx.x * 2 * len, ws + (uint64_t)blockIdx.x * 2 * len + len, len, red, spauli};
     for (uint64_t shot = blockIdx.x; shot < shots; shot += gridDim.x) {
         for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) {
