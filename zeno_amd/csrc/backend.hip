@@ -1037,6 +1037,17 @@ int backend_end_run(zb_backend *h) {
         h->la_pos = h->la_count;
         if (h->prec == 64) settle<double>(h, ~0ull);
         else settle<float>(h, ~0ull);
+        // The tile is filled up to its width anyway: its spare bits go to
+        // circuit qubits 0..low-1 that are off the low bits, so that the
+        // write-back can home them too.
+        if (h->relabel && h->la_need && h->win_ops.size() > 1) {
+            const uint32_t m = tile_bits(h), low = (uint32_t)__builtin_popcountll(low_mask(h));
+            for (uint32_t q = 0; q < low; ++q) {
+                const uint64_t next = h->win_mask | (1ull << h->phys[q]);
+                if ((uint32_t)__builtin_popcountll(next) > m) break;
+                if (page_bits_of(h, next) <= h->page_bits) h->win_mask = next;
+            }
+        }
         close_window(h);
     });
 }
