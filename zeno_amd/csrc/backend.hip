@@ -22,6 +22,13 @@
 #include "kernels.hpp"
 #include "zeno_hip.h"
 
+namespace zeno {
+// sort.hip: the sampler's draws sorted and routed to chunks on the device
+size_t draw_sort_temp_bytes(uint64_t n);
+hipError_t draw_sort_and_route(void *temp, size_t temp_bytes, const double *us, double *sorted, uint64_t n,
+                               const double *cum, uint64_t m, uint64_t *starts, hipStream_t stream);
+}  // namespace zeno
+
 using namespace zeno;
 
 namespace {
@@ -895,6 +902,11 @@ void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
 // sample_indices (src/sample.rs:29-111).  Passes 1 and 5-6 run on the
 // device; the serial prefix, the serial draws, the sort and the routing
 // (passes 2-4) run on the host, in the reference's order.
+// From this many shots the draws are sorted and routed on the device
+// (sort.hip); below it the host sort is cheaper than the launches.
+constexpr uint64_t kDeviceSortShots = 1ull << 15;
+constexpr uint64_t kDeviceSortMaxShots = 1ull << 30;  // the radix sort counts items in an int
+
 template <typename T>
 void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
     flush(h);
@@ -913,6 +925,29 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
     Rng rng(seed);
     std::vector<double> us(shots);
     for (uint64_t i = 0; i < shots; ++i) us[i] = rng.f64() * total;
+    if (shots >= kDeviceSortShots && shots <= kDeviceSortMaxShots) {
+        // sort, route and rescan on the device; every chunk gets a block
+        const size_t tb = (draw_sort_temp_bytes(shots) + 255) & ~size_t(255);
+        const size_t nb = ((nch + 1) * 8 + 255) & ~size_t(255), sb = (shots * 8 + 255) & ~size_t(255);
+        char *p = h->scratch_get(tb + 2 * nb + 3 * sb + 256);
+        p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
+        void *temp = p;
+        double *dcum = (double *)(p + tb);
+        uint64_t *dstarts = (uint64_t *)(p + tb + nb);
+        double *dus = (double *)(p + tb + 2 * nb);
+        double *dsorted = (double *)(p + tb + 2 * nb + sb);
+        uint64_t *dout = (uint64_t *)(p + tb + 2 * nb + 2 * sb);
+        check(hipMemcpyAsync(dcum, cum.data(), (nch + 1) * 8, hipMemcpyHostToDevice, h->stream), "cum");
+        check(hipMemcpyAsync(dus, us.data(), shots * 8, hipMemcpyHostToDevice, h->stream), "us");
+        check(draw_sort_and_route(temp, tb, dus, dsorted, shots, dcum, nch + 1, dstarts, h->stream), "draw sort");
+        hipLaunchKernelGGL(dev::k_rescan<T>, dim3((unsigned)nch), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
+                           h->I<T>(), h->layout(), h->len, chunk, (const uint32_t *)nullptr, dcum, dstarts, dsorted,
+                           dout);
+        check(hipGetLastError(), "k_rescan");
+        check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
+        h->sync();
+        return;
+    }
     std::sort(us.begin(), us.end());
     std::vector<uint64_t> starts(nch + 1);
     for (uint64_t c = 0; c <= nch; ++c)

@@ -977,10 +977,13 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
     __shared__ uint64_t sptr;
     __shared__ int sdone;
     __shared__ uint64_t slast[kBlock];
-    const uint32_t c = touched[blockIdx.x];
+    // touched lists the chunks with draws; without it block c is chunk c,
+    // and a chunk without draws has nothing to do
+    const uint32_t c = touched ? touched[blockIdx.x] : blockIdx.x;
     const uint64_t lo = (uint64_t)c * chunk;
     const uint64_t hi = lo + chunk < len ? lo + chunk : len;
     const uint64_t s1 = starts[c + 1];
+    if (starts[c] == s1) return;
     double acc = cum[c];
     if (threadIdx.x == 0) {
         sptr = starts[c];
