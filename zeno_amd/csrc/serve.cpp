@@ -5,9 +5,10 @@
 // mem::plan against RAM (src/mem.rs:79-130, exec.rs:261-304).  Here one HBM
 // pool is allocated once when the server starts and each admitted request
 // gets its state vector carved out of it, so admission is "does the pool
-// have room for 2 * 2^n * s bytes right now" and no request calls
-// hipMalloc.  A fixed set of worker slots each own a backend (its own HIP
-// stream, parameter arena and scratch) that is re-bound to the request's
+// have room for 2 * 2^n * s bytes right now" and no request allocates its
+// state.  A fixed set of worker slots each own a backend (its own HIP
+// stream, parameter arena and scratch; the scratch grows when a request
+// first needs more, e.g. for a large shot count) that is re-bound to the request's
 // slice of the pool; requests that arrive while others run are admitted as
 // soon as a slot and pool space are free, and their kernels run
 // concurrently with the others' on separate streams.  Admission is first
