@@ -892,18 +892,51 @@ int measure_T(zb_backend *h, uint32_t q, double u) {
     return outcome;
 }
 
+// The sampler's chunk walks are latency-bound serial chains: from 8 walks
+// per CU on, the narrow shape lets more of them share a CU.
+bool samp_narrow(uint64_t walks) {
+    static const uint64_t cus = [] {
+        int d = 0, c = 0;
+        if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&c, hipDeviceAttributeMultiprocessorCount, d) !=
+                                                   hipSuccess || c <= 0)
+            c = 256;
+        return (uint64_t)c;
+    }();
+    return walks >= 8 * cus;
+}
+
+template <typename T, typename S>
+void launch_rescan(zb_backend *h, uint64_t blocks, uint64_t chunk, const uint32_t *touched, const double *cum,
+                   const uint64_t *starts, const double *us, uint64_t *out) {
+    hipLaunchKernelGGL((dev::k_rescan<T, S>), dim3((unsigned)blocks), dim3(S::kBlock), 0, h->stream, h->R<T>(),
+                       h->I<T>(), h->layout(), h->len, chunk, touched, cum, starts, us, out);
+    check(hipGetLastError(), "k_rescan");
+}
+
+template <typename T>
+void rescan_T(zb_backend *h, uint64_t blocks, uint64_t chunk, const uint32_t *touched, const double *cum,
+              const uint64_t *starts, const double *us, uint64_t *out) {
+    if (samp_narrow(blocks)) launch_rescan<T, dev::SampNarrow>(h, blocks, chunk, touched, cum, starts, us, out);
+    else launch_rescan<T, dev::SampWide>(h, blocks, chunk, touched, cum, starts, us, out);
+}
+
 template <typename T>
 void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
-    hipLaunchKernelGGL(dev::k_chunk_sums<T>, dim3((unsigned)nch), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                       h->I<T>(), h->layout(), h->len, chunk, dsums);
+    if (samp_narrow(nch))
+        hipLaunchKernelGGL((dev::k_chunk_sums<T, dev::SampNarrow>), dim3((unsigned)nch),
+                           dim3(dev::SampNarrow::kBlock), 0, h->stream, h->R<T>(), h->I<T>(), h->layout(), h->len,
+                           chunk, dsums);
+    else
+        hipLaunchKernelGGL((dev::k_chunk_sums<T, dev::SampWide>), dim3((unsigned)nch), dim3(dev::SampWide::kBlock),
+                           0, h->stream, h->R<T>(), h->I<T>(), h->layout(), h->len, chunk, dsums);
     check(hipGetLastError(), "k_chunk_sums");
 }
 
 // sample_indices (src/sample.rs:29-111).  Passes 1 and 5-6 run on the
-// device; the serial prefix, the serial draws, the sort and the routing
-// (passes 2-4) run on the host, in the reference's order.
-// From this many shots the draws are sorted and routed on the device
-// (sort.hip); below it the host sort is cheaper than the launches.
+// device; the serial prefix and the serial draws (passes 2-3) run on the
+// host, in the reference's order.  The sort and the routing (passes 3-4)
+// run on the host below kDeviceSortShots shots and on the device (sort.hip)
+// from there, where the host sort would cost more than the launches.
 constexpr uint64_t kDeviceSortShots = 1ull << 15;
 constexpr uint64_t kDeviceSortMaxShots = 1ull << 30;  // the radix sort counts items in an int
 
@@ -940,10 +973,7 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
         check(hipMemcpyAsync(dcum, cum.data(), (nch + 1) * 8, hipMemcpyHostToDevice, h->stream), "cum");
         check(hipMemcpyAsync(dus, us.data(), shots * 8, hipMemcpyHostToDevice, h->stream), "us");
         check(draw_sort_and_route(temp, tb, dus, dsorted, shots, dcum, nch + 1, dstarts, h->stream), "draw sort");
-        hipLaunchKernelGGL(dev::k_rescan<T>, dim3((unsigned)nch), dim3(dev::kBlock), 0, h->stream, h->R<T>(),
-                           h->I<T>(), h->layout(), h->len, chunk, (const uint32_t *)nullptr, dcum, dstarts, dsorted,
-                           dout);
-        check(hipGetLastError(), "k_rescan");
+        rescan_T<T>(h, nch, chunk, nullptr, dcum, dstarts, dsorted, dout);
         check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
         h->sync();
         return;
@@ -972,9 +1002,7 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
         check(hipMemcpyAsync(dus, us.data(), shots * 8, hipMemcpyHostToDevice, h->stream), "us");
         check(hipMemcpyAsync(dtouched, touched.data(), touched.size() * 4, hipMemcpyHostToDevice, h->stream),
               "touched");
-        hipLaunchKernelGGL(dev::k_rescan<T>, dim3((unsigned)touched.size()), dim3(dev::kBlock), 0, h->stream,
-                           h->R<T>(), h->I<T>(), h->layout(), h->len, chunk, dtouched, dcum, dstarts, dus, dout);
-        check(hipGetLastError(), "k_rescan");
+        rescan_T<T>(h, touched.size(), chunk, dtouched, dcum, dstarts, dus, dout);
         check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
     }
     h->sync();

@@ -895,63 +895,123 @@ __global__ __launch_bounds__(kBlock) void k_max_abs_diff(const A *__restrict__ a
 
 // --------------------------------------------------------------- sampler
 // Lane 0's strictly ordered walk over a slab of squared moduli in LDS, in
-// register groups: two groups alternate, the loads of one issued before the
-// other's adds, so the serial f64 chain, not the LDS latency, sets the
-// pace.  group(g, v) takes the 16 values from index g and returns false to
-// stop; one(j, v) takes the tail one value at a time.
-constexpr int kSerialGroup = 16;
-
+// register groups of G values: two groups alternate, the loads of one
+// issued before the other's adds, so the serial f64 chain, not the LDS
+// latency, sets the pace.  group(g, v) takes the G values from index g and
+// returns false to stop; one(j, v) takes the tail one value at a time.
+template <int G>
 __device__ __forceinline__ void load_group(double *v, const double *tv) {
 #pragma unroll
-    for (int u = 0; u < kSerialGroup; ++u) v[u] = tv[u];
+    for (int u = 0; u < G; ++u) v[u] = tv[u];
 }
 
-template <typename Group, typename One>
+template <int G, typename Group, typename One>
 __device__ __forceinline__ void serial_walk(const double *tv, int cnt, Group &&group, One &&one) {
-    const int full = cnt / kSerialGroup * kSerialGroup;
-    double a[kSerialGroup], b[kSerialGroup];
-    if (full > 0) load_group(a, tv);
+    const int full = cnt / G * G;
+    double a[G], b[G];
+    if (full > 0) load_group<G>(a, tv);
     for (int g = 0; g < full;) {
-        const bool more_b = g + kSerialGroup < full;
-        if (more_b) load_group(b, tv + g + kSerialGroup);
+        const bool more_b = g + G < full;
+        if (more_b) load_group<G>(b, tv + g + G);
         if (!group(g, a)) return;
-        g += kSerialGroup;
+        g += G;
         if (!more_b) break;
-        const bool more_a = g + kSerialGroup < full;
-        if (more_a) load_group(a, tv + g + kSerialGroup);
+        const bool more_a = g + G < full;
+        if (more_a) load_group<G>(a, tv + g + G);
         if (!group(g, b)) return;
-        g += kSerialGroup;
+        g += G;
         if (!more_a) break;
     }
     for (int j = full; j < cnt; ++j)
         if (!one(j, tv[j])) return;
 }
 
+// Shapes of the sampler's chunk walks (one workgroup per chunk; lane 0's
+// serial chain sets the time): with few chunks, 4 waves load 4096-value
+// slabs just before squaring them and lane 0 walks 16-value groups, the
+// shortest walk per chunk at a register count that lets several share a CU;
+// with many chunks, one wave per chunk with 512-value slabs, loaded while
+// lane 0 walks the previous one, and 8-value groups keeps registers and LDS
+// small, so many chains share a CU.
+template <int BS, int SPAN, int G, bool PF>
+struct SampShape {
+    static constexpr int kBlock = BS, kSpan = SPAN, kGroup = G;
+    static constexpr bool kPrefetch = PF;
+    static_assert(SPAN % BS == 0 && SPAN % G == 0, "whole lanes and groups per slab");
+};
+using SampWide = SampShape<256, 4096, 16, false>;
+using SampNarrow = SampShape<64, 512, 8, true>;
+
+// A workgroup's slab of amplitudes in circuit order: each lane holds
+// S::kSpan / S::kBlock of them in registers (with S::kPrefetch, loaded while
+// lane 0 still walks the previous slab) and squares them into LDS (r*r + i*i
+// in f64).
+template <typename T, typename S>
+struct SampSlab {
+    static constexpr int kPerLane = S::kSpan / S::kBlock;
+    T r[kPerLane], m[kPerLane];
+    __device__ __forceinline__ void load(const T *__restrict__ re, const T *__restrict__ im, const Layout &L,
+                                         uint64_t slab, int cnt) {
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) {
+            const int j = threadIdx.x + u * S::kBlock;
+            if (j < cnt) {
+                const uint64_t p = phys_index(L, slab + j);
+                r[u] = re[p];
+                m[u] = im[p];
+            }
+        }
+    }
+    // Without prefetch: load, square and store one value at a time.
+    __device__ __forceinline__ static void fill(const T *__restrict__ re, const T *__restrict__ im, const Layout &L,
+                                                uint64_t slab, int cnt, double *tv) {
+        for (int j = threadIdx.x; j < cnt; j += S::kBlock) {
+            const uint64_t p = phys_index(L, slab + j);
+            const double a = (double)re[p], b = (double)im[p];
+            tv[j] = a * a + b * b;
+        }
+    }
+    __device__ __forceinline__ void store(double *tv, int cnt) const {
+#pragma unroll
+        for (int u = 0; u < kPerLane; ++u) {
+            const int j = threadIdx.x + u * S::kBlock;
+            if (j < cnt) {
+                const double a = (double)r[u], b = (double)m[u];
+                tv[j] = a * a + b * b;
+            }
+        }
+    }
+};
+
 // sample_indices pass 1 (src/sample.rs:38-50): per chunk a strictly serial
-// f64 sum acc += r*r + i*i in index order.  The 256 lanes load a 4096
-// element slab coalesced and square it into LDS; lane 0 then accumulates
-// the slab serially, so the sum order is exactly the reference's.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re, const T *__restrict__ im, Layout L,
+// f64 sum acc += r*r + i*i in index order.  The lanes load a slab coalesced
+// and square it into LDS; lane 0 then accumulates the slab serially, so the
+// sum order is exactly the reference's (with S::kPrefetch while the lanes'
+// loads of the next slab are in flight).
+template <typename T, typename S>
+__global__ __launch_bounds__(S::kBlock) void k_chunk_sums(const T *__restrict__ re, const T *__restrict__ im, Layout L,
                                                        uint64_t len, uint64_t chunk, double *__restrict__ sums) {
-    __shared__ double tv[kReduceSpan];
+    __shared__ double tv[S::kSpan];
     const uint64_t lo = (uint64_t)blockIdx.x * chunk;
     const uint64_t hi = lo + chunk < len ? lo + chunk : len;
+    auto count = [&](uint64_t slab) {
+        return (int)(hi - slab < (uint64_t)S::kSpan ? hi - slab : (uint64_t)S::kSpan);
+    };
     double acc = 0.0;
-    for (uint64_t slab = lo; slab < hi; slab += kReduceSpan) {
-        const int cnt = (int)(hi - slab < (uint64_t)kReduceSpan ? hi - slab : (uint64_t)kReduceSpan);
-        for (int j = threadIdx.x; j < cnt; j += kBlock) {
-            const uint64_t p = phys_index(L, slab + j);
-            const double r = (double)re[p], m = (double)im[p];
-            tv[j] = r * r + m * m;
-        }
+    SampSlab<T, S> w;
+    if (S::kPrefetch && lo < hi) w.load(re, im, L, lo, count(lo));
+    for (uint64_t slab = lo; slab < hi; slab += S::kSpan) {
+        const int cnt = count(slab);
+        if (S::kPrefetch) w.store(tv, cnt);
+        else SampSlab<T, S>::fill(re, im, L, slab, cnt, tv);
         __syncthreads();
+        if (S::kPrefetch && slab + S::kSpan < hi) w.load(re, im, L, slab + S::kSpan, count(slab + S::kSpan));
         if (threadIdx.x == 0)
-            serial_walk(
+            serial_walk<S::kGroup>(
                 tv, cnt,
                 [&](int, const double *v) {
 #pragma unroll
-                    for (int u = 0; u < kSerialGroup; ++u) acc += v[u];
+                    for (int u = 0; u < S::kGroup; ++u) acc += v[u];
                     return true;
                 },
                 [&](int, double v) {
@@ -967,16 +1027,16 @@ __global__ __launch_bounds__(kBlock) void k_chunk_sums(const T *__restrict__ re,
 // chunk from cum_before[c], emitting the sorted draws us[s0..s1) that fall
 // below the running sum; leftovers land on the chunk's last index with
 // nonzero probability.
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, Layout L,
+template <typename T, typename S>
+__global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, Layout L,
                                                    uint64_t len,
                                                    uint64_t chunk, const uint32_t *__restrict__ touched,
                                                    const double *__restrict__ cum, const uint64_t *__restrict__ starts,
                                                    const double *__restrict__ us, uint64_t *__restrict__ out) {
-    __shared__ double tv[kReduceSpan];
+    __shared__ double tv[S::kSpan];
     __shared__ uint64_t sptr;
     __shared__ int sdone;
-    __shared__ uint64_t slast[kBlock];
+    __shared__ uint64_t slast[S::kBlock];
     // touched lists the chunks with draws; without it block c is chunk c,
     // and a chunk without draws has nothing to do
     const uint32_t c = touched ? touched[blockIdx.x] : blockIdx.x;
@@ -990,14 +1050,17 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
         sdone = 0;
     }
     __syncthreads();
-    for (uint64_t slab = lo; slab < hi; slab += kReduceSpan) {
-        const int cnt = (int)(hi - slab < (uint64_t)kReduceSpan ? hi - slab : (uint64_t)kReduceSpan);
-        for (int j = threadIdx.x; j < cnt; j += kBlock) {
-            const uint64_t p = phys_index(L, slab + j);
-            const double r = (double)re[p], m = (double)im[p];
-            tv[j] = r * r + m * m;
-        }
+    auto count = [&](uint64_t slab) {
+        return (int)(hi - slab < (uint64_t)S::kSpan ? hi - slab : (uint64_t)S::kSpan);
+    };
+    SampSlab<T, S> w;
+    if (S::kPrefetch) w.load(re, im, L, lo, count(lo));
+    for (uint64_t slab = lo; slab < hi; slab += S::kSpan) {
+        const int cnt = count(slab);
+        if (S::kPrefetch) w.store(tv, cnt);
+        else SampSlab<T, S>::fill(re, im, L, slab, cnt, tv);
         __syncthreads();
+        if (S::kPrefetch && slab + S::kSpan < hi) w.load(re, im, L, slab + S::kSpan, count(slab + S::kSpan));
         if (threadIdx.x == 0) {
             // The next draw's threshold stays in a register, reloaded only
             // when a draw is emitted (+inf once all are).  acc never
@@ -1012,16 +1075,16 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
                     nu = ptr < s1 ? us[ptr] : __builtin_inf();
                 }
             };
-            serial_walk(
+            serial_walk<S::kGroup>(
                 tv, cnt,
                 [&](int g, const double *v) {
-                    double run[kSerialGroup], x = acc;
+                    double run[S::kGroup], x = acc;
 #pragma unroll
-                    for (int u = 0; u < kSerialGroup; ++u) run[u] = x = x + v[u];
+                    for (int u = 0; u < S::kGroup; ++u) run[u] = x = x + v[u];
                     acc = x;
                     if (nu < x) {
 #pragma unroll
-                        for (int u = 0; u < kSerialGroup; ++u) emit(g + u, run[u]);
+                        for (int u = 0; u < S::kGroup; ++u) emit(g + u, run[u]);
                     }
                     return ptr < s1;
                 },
@@ -1040,7 +1103,7 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
     // Fallback: last index of the chunk with nonzero probability (else hi-1).
     uint64_t best = 0;
     int found = 0;
-    for (uint64_t i = lo + threadIdx.x; i < hi; i += kBlock) {
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += S::kBlock) {
         const uint64_t p = phys_index(L, i);
         const double r = (double)re[p], m = (double)im[p];
         if (r * r + m * m > 0.0) best = i, found = 1;
@@ -1049,7 +1112,7 @@ __global__ __launch_bounds__(kBlock) void k_rescan(const T *__restrict__ re, con
     __syncthreads();
     if (threadIdx.x == 0) {
         uint64_t m = 0;
-        for (int t = 0; t < kBlock; ++t) m = slast[t] > m ? slast[t] : m;
+        for (int t = 0; t < S::kBlock; ++t) m = slast[t] > m ? slast[t] : m;
         const uint64_t last = m ? m - 1 : hi - 1;
         for (uint64_t p = sptr; p < s1; ++p) out[p] = last;
     }
