@@ -335,6 +335,29 @@ def test_sampler_indices_and_chunk_sums_bit_exact(orc):
             assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed))
 
 
+def test_sampler_many_chunks_bit_exact(orc):
+    """At 27 qubits (2048 chunks, 8 per CU) the chunk walks take their
+    one-wave shape, and so does the rescan of a 100000-shot sample (device
+    sort, a block per chunk); 1000 shots rescan the touched chunks with the
+    four-wave shape.  Chunk sums and draws equal the oracle's."""
+    n = 27
+    prog = orc.gen_random(n, 2, 2700)
+    orc.set_threads(16)
+    try:
+        for prec in (32, 64):
+            _say(f"sampler many chunks n={n} prec={prec}")
+            o = orc.State(n, prec)
+            o.apply_compiled(orc.Compiled(prog, n, fusion=2))
+            with z.HipBackend(n, prec) as g:
+                g.apply_compiled(z.compile(prog, n, fusion=2))
+                assert np.array_equal(g.chunk_sums().view(np.uint64), o.chunk_sums().view(np.uint64))
+                for shots, seed in ((100000, 5), (1000, 6)):
+                    assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed)), (prec, shots)
+            del o
+    finally:
+        orc.set_threads(1)
+
+
 def test_sampler_concentrated_distributions_bit_exact(orc):
     """Mass on a few indices: GHZ (index 0 and the last index, every chunk
     between empty) and H on qubits 0..2 only (8 indices inside the first
