@@ -104,7 +104,9 @@ int zb_reset_all(zb_backend *h);
 
 /* Backend::sample (src/exec.rs:154; src/sample.rs:29-111) with
  * rng = Xoshiro256PlusPlus::seed_from_u64(seed).  Writes `shots` basis
- * indices in ascending order to `out` (host memory). */
+ * indices in ascending order to `out` (host memory).  The handle's device
+ * scratch grows to hold the draws (about 24 bytes per shot from 32768
+ * shots, when they are sorted on the device). */
 int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out);
 
 /* Backend::statevector (src/exec.rs:155; src/state.rs:134-140): writes
