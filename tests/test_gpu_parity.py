@@ -627,6 +627,30 @@ def test_serving_results_same_alone_and_in_mix(orc):
         assert overlap > 0, "no two requests ran at the same time"
 
 
+def test_serving_large_requests_one_at_a_time():
+    """Requests of 1 GiB of state and more run one at a time while smaller
+    ones share the other slots; every result equals the request run alone."""
+    large = z.compile(z.Circuit(26).extend(z.random_circuit(26, 6, 3).to_program()[0]).measure_all().to_program()[0],
+                      26, fusion=1)
+    small = z.compile(z.Circuit(16).extend(z.random_circuit(16, 20, 4).to_program()[0]).measure_all().to_program()[0],
+                      16, fusion=1)
+    plan = [(large, 11), (small, 12), (large, 13), (small, 14), (small, 15), (large, 16), (small, 17)]
+    alone = [z.run_compiled(c, z.RunOptions(shots=500, seed=s, precision=64)).raw_counts for c, s in plan]
+    _say("serving large one at a time")
+    with z.Server(5 << 30, 4) as srv:
+        ids = [srv.submit(c, 64, 500, s) for c, s in plan]
+        got = [srv.wait(i) for i in ids]
+    for g, a in zip(got, alone):
+        assert g.raw_counts == a
+    big = [g for g, (c, _) in zip(got, plan) if c is large]
+    for a in big:
+        for b in big:
+            if a.id < b.id:
+                assert a.t_end <= b.t_start or b.t_end <= a.t_start, "two large requests overlapped"
+    lit = [g for g, (c, _) in zip(got, plan) if c is small]
+    assert any(s.t_start < b.t_end and b.t_start < s.t_end for s in lit for b in big), "small ones waited"
+
+
 def test_serving_admission_limits():
     with z.Server(64 << 20, 2) as srv:
         big = z.compile(z.random_circuit(23, 2, 1).to_program()[0], 23)

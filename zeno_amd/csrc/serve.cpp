@@ -14,7 +14,8 @@
 // concurrently with the others' on separate streams.  Admission is first
 // fit in arrival order, except that a request that cannot fit is passed
 // over at most kMaxSkips times before the server stops admitting behind
-// it, so large requests are not starved by a stream of small ones.
+// it, so large requests are not starved by a stream of small ones, and that
+// requests of 1 GiB of state and more run one at a time.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -38,6 +39,10 @@ namespace {
 
 constexpr uint64_t kAlign = 1ull << 16;
 constexpr int kMaxSkips = 16;
+// Requests whose state is at least this large run one at a time: each one's
+// sweeps stream the whole chip's HBM, so two side by side both finish later.
+// Smaller requests keep the other slots meanwhile.
+constexpr uint64_t kLargeBytes = 1ull << 30;
 
 using Clock = std::chrono::steady_clock;
 
@@ -118,7 +123,7 @@ struct zs_server {
     std::unordered_map<int64_t, std::unique_ptr<Request>> requests;
     std::vector<std::thread> workers;
     int64_t next_id = 1;
-    int running = 0;
+    int running = 0, running_large = 0;
     bool stop = false;
 
     double now() const { return std::chrono::duration<double>(Clock::now() - epoch).count(); }
@@ -127,6 +132,7 @@ struct zs_server {
     Request *admit() {
         for (size_t i = 0; i < pending.size(); ++i) {
             Request *r = pending[i];
+            if (r->bytes >= kLargeBytes && running_large > 0) continue;  // waits for the running one, not starved
             uint64_t off = 0;
             if (pool->alloc(r->bytes, &off)) {
                 r->off = off;
@@ -146,13 +152,16 @@ struct zs_server {
             Request *r = nullptr;
             cv_work.wait(lk, [&] { return stop || (r = admit()) != nullptr; });
             if (!r) break;
+            const bool large = r->bytes >= kLargeBytes;
             ++running;
+            running_large += large;
             r->t_start = now();
             lk.unlock();
             run(h, r);
             lk.lock();
             pool->release(r->off);
             --running;
+            running_large -= large;
             r->t_end = now();
             r->done = true;
             cv_done.notify_all();
