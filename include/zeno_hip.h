@@ -6,7 +6,7 @@
  *  1. zb_*  — the drop-in boundary.  One entry point per method of the
  *             reference's `pub trait Backend: Send` (src/exec.rs:134-162)
  *             plus its constructor make_backend (src/exec.rs:306-331).  A
- *             Rust FFI crate binds these 1:1 (binding in INTEGRATION.md).
+ *             Rust FFI crate binds these 1:1.
  *  2. zr_*  — the host side above the boundary (compiler + executor),
  *             mirroring src/compiler.rs:118-200 compile() and
  *             src/exec.rs:334-452 run_compiled/run_sampled.  The Python

@@ -393,12 +393,12 @@ def test_measure_and_collapse_bit_exact(orc):
         g.close()
 
 
-def test_c1_counts_identical(orc):
+def test_20_qubit_counts_identical(orc):
     """20 qubits, depth 12, fusion 2, 1024 shots, seed 42 — both the
     reference brickwork and the random-matching variant."""
     for prog in (orc.gen_random(20, 12, 42), orc.gen_matching(20, 12, 42)):
         full = orc.program([prog, orc.measure_all(20)])
-        _say("C1 counts")
+        _say("20q d12 counts")
         want, _ = orc.run(orc.Compiled(full, 20, fusion=2), 1024, 42)
         r = z.run_program(full, 20, opts=z.RunOptions(shots=1024, seed=42, precision=64, fusion_max=2))
         assert r.raw_counts == want
@@ -547,30 +547,30 @@ def _run_pair(orc, prog, n, fusion, prec, shots, seed):
         assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed))
 
 
-def test_c2_qft24_and_random26_bit_exact_with_1m_shots(orc):
-    """QFT on 24 qubits after a seeded U3 layer, and the
-    random circuit at 26 qubits depth 12, fusion K in {2, 3, 4}, f64, 1M shots."""
+def test_qft24_and_random26_bit_exact_with_1m_shots(orc):
+    """QFT on 24 qubits after a seeded U3 layer, and the random circuit at 26
+    qubits depth 12, fusion K in {2, 3, 4}, f64, 1M shots."""
     orc.set_threads(16)
     try:
         for fusion in (2, 3, 4):
-            _say(f"C2 qft24 fusion={fusion}")
+            _say(f"qft24 fusion={fusion}")
             _run_pair(orc, orc.gen_u3_qft(24, 11), 24, fusion, 64, 1_000_000, 42)
         for fusion in (2, 3, 4):
-            _say(f"C2 random26 fusion={fusion}")
+            _say(f"random26 fusion={fusion}")
             _run_pair(orc, orc.gen_random(26, 12, 26), 26, fusion, 64, 1_000_000, 7)
     finally:
         orc.set_threads(1)
 
 
-def test_c3_33_qubit_state_roundtrip():
-    """BASELINE config C3's large state: 33 qubits (128 GiB f64, 64 GiB f32),
-    matching circuit with RZZ ladders, forward then adjoint returns |0...0>."""
+def test_33_qubit_state_roundtrip():
+    """The largest state: 33 qubits (128 GiB f64, 64 GiB f32), matching
+    circuit with RZZ ladders, forward then adjoint returns |0...0>."""
     free, _ = z.device_memory()
     n = 33
     for prec, tol in ((64, 1e-9), (32, 1e-4)):
         if free < (1 << n) * (16 if prec == 64 else 8) + (8 << 30):
             pytest.skip("not enough device memory for a 33-qubit state")
-        _say(f"C3 33q roundtrip prec={prec}")
+        _say(f"33q roundtrip prec={prec}")
         _adjoint_roundtrip(n, prec, z.matching_circuit(n, 5, 3, 5).to_program()[0], 1, tol)
 
 
