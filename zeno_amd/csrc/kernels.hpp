@@ -1062,17 +1062,22 @@ __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, 
         __syncthreads();
         if (S::kPrefetch && slab + S::kSpan < hi) w.load(re, im, L, slab + S::kSpan, count(slab + S::kSpan));
         if (threadIdx.x == 0) {
-            // The next draw's threshold stays in a register, reloaded only
-            // when a draw is emitted (+inf once all are).  acc never
-            // decreases (it adds r*r + i*i >= 0 and rounding is monotonic),
-            // so a group whose last running sum does not pass the threshold
-            // emits nothing and needs no per-element test.
+            // The next four draws' thresholds stay in registers (+inf past
+            // the chunk's last draw): an emission shifts them and loads the
+            // one four ahead, so a run of emissions does not wait on memory.
+            // acc never decreases (it adds r*r + i*i >= 0 and rounding is
+            // monotonic), so a group whose last running sum does not pass
+            // the next threshold emits nothing and needs no per-element test.
             uint64_t ptr = sptr;
-            double nu = ptr < s1 ? us[ptr] : __builtin_inf();
+            auto draw = [&](uint64_t i) { return i < s1 ? us[i] : __builtin_inf(); };
+            double nu = draw(ptr), nu1 = draw(ptr + 1), nu2 = draw(ptr + 2), nu3 = draw(ptr + 3);
             auto emit = [&](int j, double x) {
                 while (nu < x) {
                     out[ptr++] = slab + (uint64_t)j;
-                    nu = ptr < s1 ? us[ptr] : __builtin_inf();
+                    nu = nu1;
+                    nu1 = nu2;
+                    nu2 = nu3;
+                    nu3 = draw(ptr + 3);
                 }
             };
             serial_walk<S::kGroup>(
