@@ -567,6 +567,21 @@ __device__ __forceinline__ void reg_cx_slot(T *xr, T *xi) {
     }
 }
 
+// CX whose control is fixed for the whole tile and set: swap every pair.
+template <typename T, int NS, int ST>
+__device__ __forceinline__ void reg_cx_swap(T *xr, T *xi) {
+#pragma unroll
+    for (int r = 0; r < (1 << NS); ++r) {
+        if ((r >> ST) & 1) continue;
+        const int c = r | (1 << ST);
+        const T a0 = xr[r], a1 = xi[r];
+        xr[r] = xr[c];
+        xi[r] = xi[c];
+        xr[c] = a0;
+        xi[c] = a1;
+    }
+}
+
 template <typename T, int NS, int ST>
 __device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
 #pragma unroll
@@ -763,7 +778,8 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                 // register state twice and the merges cost register copies
                 constexpr uint32_t k1 = kW1q << 8, kc = kWCx << 8, ks = kCtrlSlot << 4, kl = kCtrlLocal << 4,
                                    kf = kCtrlFixed << 4;
-                const bool on = (sel >> 4) == kCtrlLocal ? ((lb >> op->aux) & 1) : ((base >> op->aux) & 1);
+                const bool on = (lb >> op->aux) & 1;               // kCtrlLocal
+                const bool fixed_on = (base >> op->aux) & 1;       // kCtrlFixed: uniform
                 switch (kind << 8 | sel) {
                     case k1 | 0: reg_1q<T, NS, 0, F>(xr, xi, mats + op->data_off); break;
                     case k1 | 1: reg_1q<T, NS, 1, F>(xr, xi, mats + op->data_off); break;
@@ -781,10 +797,16 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
                     case kc | ks | 3 | 0 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 0>(xr, xi); break;
                     case kc | ks | 3 | 1 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 1>(xr, xi); break;
                     case kc | ks | 3 | 2 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 2>(xr, xi); break;
-                    case kc | kl | 0: case kc | kf | 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
-                    case kc | kl | 1: case kc | kf | 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
-                    case kc | kl | 2: case kc | kf | 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
-                    case kc | kl | 3: case kc | kf | 3: if constexpr (NS > 3) reg_cx_all<T, NS, 3>(xr, xi, on); break;
+                    // control on a lane bit: per-lane selects; fixed for the
+                    // tile: a uniform branch around plain swaps
+                    case kc | kl | 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
+                    case kc | kl | 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
+                    case kc | kl | 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
+                    case kc | kl | 3: if constexpr (NS > 3) reg_cx_all<T, NS, 3>(xr, xi, on); break;
+                    case kc | kf | 0: if (fixed_on) reg_cx_swap<T, NS, 0>(xr, xi); break;
+                    case kc | kf | 1: if (fixed_on) reg_cx_swap<T, NS, 1>(xr, xi); break;
+                    case kc | kf | 2: if constexpr (NS > 2) if (fixed_on) reg_cx_swap<T, NS, 2>(xr, xi); break;
+                    case kc | kf | 3: if constexpr (NS > 3) if (fixed_on) reg_cx_swap<T, NS, 3>(xr, xi); break;
                     default:
                         if (kind == kWDense) {
                             if constexpr (kDense) reg_dense_dispatch<T, NS, F>(xr, xi, sel, mats + op->data_off);
