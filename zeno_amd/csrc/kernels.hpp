@@ -684,8 +684,13 @@ __attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_wi
             woff[j] = padded(u);
         }
     }
+    // Workgroups are dealt to the 8 XCDs in turn, so workgroups w and w + 8
+    // share an XCD (and its L2) and start together: within each 16 tiles
+    // they take tiles 2i and 2i + 1, which differ in the lowest state bit
+    // outside the tile and may share cache lines.
+    const bool pair = tiles % 16 == 0;
     auto tile_base = [&](uint64_t tile) {
-        uint64_t b = tile;
+        uint64_t b = pair ? (tile & ~15ull) | ((tile & 7) << 1) | ((tile >> 3) & 1) : tile;
 #pragma unroll
         for (int i = 0; i < TB; ++i) b = insert_zero(b, hdr->q[i]);
         return b;
