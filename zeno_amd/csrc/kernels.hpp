@@ -631,9 +631,14 @@ struct WinShape {
     static constexpr int kLdsBytes = 2 * kTilePad * (int)sizeof(T);
     static constexpr int kPerCu = (160 * 1024) / kLdsBytes < 1 ? 1 : (160 * 1024) / kLdsBytes;
     // occupancy asked of the compiler: two workgroups per CU for 12-bit
-    // tiles, one for 13-bit ones, at most 4 waves per SIMD (128 VGPRs)
+    // tiles, one for 13-bit ones, at most 4 waves per SIMD (128 VGPRs) ...
     static constexpr int kWant = (TB == 12 ? 2 : 1) * kLanes / 256;
     static constexpr int kWavesPerEu = kWant < 1 ? 1 : (kWant > 4 ? 4 : kWant);
+    // ... or, for the 3-slot windows without dense ops (which fit without
+    // spilling), as many workgroups per CU as the LDS holds, at most 8 waves
+    // per SIMD (f32: 4 workgroups, 64 VGPRs)
+    static constexpr int kWantLds = kPerCu * (kLanes / 64) / 4;
+    static constexpr int kWavesPerEuLight = kWantLds < kWavesPerEu ? kWavesPerEu : (kWantLds > 8 ? 8 : kWantLds);
 };
 
 // kDense: the window holds dense k >= 2 ops; kPrefetch: the next tile's HBM
@@ -643,7 +648,8 @@ struct WinShape {
 // base | hi(j); a lane adds its offset lo (its low local bits).
 template <typename T, int TB, int NS, bool kDense, bool kPrefetch, bool F>
 __global__ __attribute__((amdgpu_flat_work_group_size(1, WinShape<T, TB, NS>::kLanes)))
-__attribute__((amdgpu_waves_per_eu(WinShape<T, TB, NS>::kWavesPerEu))) void k_window(
+__attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWavesPerEuLight
+                                                       : WinShape<T, TB, NS>::kWavesPerEu))) void k_window(
     T *__restrict__ re, T *__restrict__ im, const uint32_t *__restrict__ blob, const T *__restrict__ tables,
     uint64_t tiles) {
     using Shape = WinShape<T, TB, NS>;
