@@ -768,7 +768,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
             for (int i = 0; i < NS; ++i) sl[i] = (int)st->slot[i];
             int lb = t;
 #pragma unroll
-            for (int i = 0; i < NS; ++i) lb = (int)insert_zero((uint64_t)lb, sl[i]);
+            for (int i = 0; i < NS; ++i) lb += lb & -(1 << sl[i]);  // insert a zero bit at sl[i]
             const int plb = padded(lb);
             T xr[R], xi[R];
             // lb and the register bits are disjoint, so padded(lb | c) =
