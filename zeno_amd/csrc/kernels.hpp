@@ -709,14 +709,18 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
         return h;
     };
     T pr[R], pi[R];
+    // The lane's offset lo and a row's uniform index have disjoint bits, so
+    // element row | lo is at (array + lo) + row: one uniform add per access.
+    T *__restrict__ const rel = re + lo;
+    T *__restrict__ const iml = im + lo;
     uint64_t tile = blockIdx.x;
     uint64_t base = tile < tiles ? tile_base(tile) : 0;
     if (kPrefetch && tile < tiles) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            const uint64_t g = row(base, j) | lo;
-            pr[j] = re[g];
-            pi[j] = im[g];
+            const uint64_t g = row(base, j);
+            pr[j] = rel[g];
+            pi[j] = iml[g];
         }
     }
     // Write-back registers of the previous tile.  Its global stores are
@@ -730,9 +734,9 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
         if constexpr (!kPrefetch) {
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                const uint64_t g = row(base, j) | lo;
-                pr[j] = re[g];
-                pi[j] = im[g];
+                const uint64_t g = row(base, j);
+                pr[j] = rel[g];
+                pi[j] = iml[g];
             }
         }
 #pragma unroll
@@ -743,9 +747,9 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
         if (pending) {
 #pragma unroll
             for (int j = 0; j < R; ++j) {
-                const uint64_t g = row(wbase, j) | lo;
-                re[g] = wr[j];
-                im[g] = wi[j];
+                const uint64_t g = row(wbase, j);
+                rel[g] = wr[j];
+                iml[g] = wi[j];
             }
         }
         __syncthreads();
@@ -755,9 +759,9 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
             if (next < tiles) {
 #pragma unroll
                 for (int j = 0; j < R; ++j) {
-                    const uint64_t g = row(nbase, j) | lo;
-                    pr[j] = re[g];
-                    pi[j] = im[g];
+                    const uint64_t g = row(nbase, j);
+                    pr[j] = rel[g];
+                    pi[j] = iml[g];
                 }
             }
         }
@@ -884,9 +888,9 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
     if (pending) {
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            const uint64_t g = row(wbase, j) | lo;
-            re[g] = wr[j];
-            im[g] = wi[j];
+            const uint64_t g = row(wbase, j);
+            rel[g] = wr[j];
+            iml[g] = wi[j];
         }
     }
 }
