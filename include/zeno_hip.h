@@ -70,7 +70,9 @@ int zb_create_bound(uint32_t n_qubits, int precision, void *re, void *im,
 /* Re-point a handle at other caller-owned state memory, possibly with a
  * different qubit count and precision (finishes queued work first, frees
  * a state the handle owned, sets |0...0>); the stream, parameter arena and
- * scratch buffers are kept. */
+ * scratch buffers are kept.  Every zb_tune knob returns to the library
+ * default and then takes the zb_tune_default values of the new shape, as
+ * at zb_create: nothing set for an earlier shape carries over. */
 int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re,
               void *im);
 
@@ -264,6 +266,40 @@ int64_t zr_compiled_final_measures(const zr_compiled *c, uint32_t *pairs);
  * (src/exec.rs:398-405 apply_op in a loop). */
 int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi);
 
+/* mem::MemPlan (src/mem.rs:95-101) together with the fields of
+ * Error::Memory (src/lib.rs:82-95), so a binding can build either. */
+typedef struct {
+    uint32_t n_qubits;
+    int32_t precision;     /* 32 or 64 */
+    uint64_t needed_bytes; /* mem::state_bytes: 2^n * (8 or 16), saturated */
+    uint64_t budget_bytes;
+    uint32_t max_qubits;   /* mem::max_qubits(budget, precision) */
+} zr_mem_plan;
+
+/* mem::budget_bytes (src/mem.rs:103-109) with HBM in place of RAM:
+ * mem_limit > 0 wins; otherwise fraction * the device's total HBM
+ * (hipMemGetInfo), where a numeric ZENO_MEM_BYTES replaces that total as it
+ * replaces physical RAM in src/mem.rs:11-17.  Returns 0 when there is no
+ * override and no device. */
+uint64_t zr_budget_bytes(uint64_t mem_limit, double fraction);
+
+/* mem::plan (src/mem.rs:114-130): fills *plan (may be NULL); ZB_OK when
+ * the state fits, else ZB_ERR_MEMORY with Error::Memory's message
+ * ("state vector for N qubits at fP needs X but the budget is Y (max M
+ * qubits at this precision; ...)") in zb_last_error(). */
+int zr_plan(uint32_t n_qubits, int precision, uint64_t budget,
+            zr_mem_plan *plan);
+
+/* resolve_precision (src/exec.rs:261-304) on the HBM budget.  requested:
+ * 0 = auto (f64 if it fits, else f32 with the reference's auto-precision
+ * notice written to `notice`, NUL-terminated, at most notice_cap bytes),
+ * 32 or 64 = that precision if it fits.  *precision gets the choice and
+ * *plan its mem::plan; ZB_ERR_MEMORY (with *plan filled) when even the
+ * chosen precision does not fit. */
+int zr_resolve_precision(uint32_t n_qubits, int requested, uint64_t budget,
+                         int *precision, zr_mem_plan *plan, char *notice,
+                         size_t notice_cap);
+
 /* exec::run_compiled (src/exec.rs:334-549) on the HIP backend: static
  * circuits run once and are sampled (run_sampled); dynamic or noisy ones
  * run per shot (run_per_shot).  Writes the unique clbit words with their
@@ -282,11 +318,29 @@ int64_t zr_run(const zr_compiled *c, int precision, uint64_t shots,
  * splitmix64(seed ^ s) and the frozen draw order, so out[s] (the shot's
  * clbit word) equals the CPU per-shot path's.  Needs n <= 20 and
  * unitary/diagonal ops on <= 3 qubits (zr_trajectories_supported);
- * kernel_seconds gets the launch-to-completion time. */
+ * kernel_seconds gets the launch-to-completion time.  The call allocates
+ * one workspace for up to two concurrent trajectories per CU, fewer when
+ * free device memory is short (ZB_ERR_MEMORY when not even one fits). */
 int zr_trajectories_supported(const zr_compiled *c);
 int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots,
                         uint64_t seed, const zr_noise *noise, uint64_t *out,
                         double *kernel_seconds);
+
+/* Many small static circuits as uneven instances of one persistent kernel
+ * launch (no counterpart in the reference, which runs one circuit per
+ * call; each instance reproduces run_sampled, src/exec.rs:407-452).  Each
+ * instance's state stays in on-chip memory for its whole run: static
+ * circuits of n <= 13 qubits in f64 or <= 14 in f32, dense ops on <= 3
+ * qubits in f64 or <= 4 in f32, <= 2^17 shots (zr_batch_supported).  The launch is captured in a
+ * hipGraph.  words gets, for instance i at offset shots[0] + ... +
+ * shots[i-1], its shots' clbit words (instances without final measures
+ * write none: their counts are empty); their counts equal zr_run's for the
+ * same circuit, precision, shots and seed.  seconds (may be NULL) gets the
+ * wall time of the batch. */
+int zr_batch_supported(const zr_compiled *c, int precision, uint64_t shots);
+int zr_run_batch(const zr_compiled *const *circuits, const int32_t *precisions,
+                 const uint64_t *shots, const uint64_t *seeds, int64_t count,
+                 uint64_t *words, double *seconds);
 
 /* ---------------------------------------------------------------------
  * Layer 3: serving loop (no counterpart in the reference, which runs one
@@ -298,7 +352,11 @@ int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots,
  * a backend with its own HIP stream.  A request is admitted, first fit in
  * arrival order, when a slot is idle and the pool has room for its state
  * (2 * 2^n * 4 or 8 bytes); its state lives in the pool until it
- * completes.  mode: see zb_set_mode. */
+ * completes.  A noisy or dynamic request whose shots run as batched
+ * trajectories is charged its trajectory workspace instead when that is
+ * larger (up to two states per CU, at most 2 GiB of them, plus op tables),
+ * and runs in it: no request allocates device memory.  mode: see
+ * zb_set_mode. */
 typedef struct zs_server zs_server;
 int zs_create(uint64_t pool_bytes, int slots, int mode, zs_server **out);
 /* Queue a run of a compiled circuit (the server copies it; the caller may
@@ -314,6 +372,15 @@ int64_t zs_wait(zs_server *s, int64_t id, uint64_t *keys, uint64_t *counts,
 /* out[4] = queued requests, running requests, largest free pool block,
  * pool size (bytes). */
 void zs_status(zs_server *s, int64_t *out);
+/* Small static requests (zr_batch_supported) do not take a slot: a batcher
+ * runs all that have arrived, up to max_batch (default 1024), as instances
+ * of one persistent launch while the slots run the others; whatever arrives
+ * meanwhile forms the next batch.  max_batch 0 sends every request to the
+ * slots (one at a time with slots = 1).  Exact mode (mode 0) never batches. */
+int zs_set_batch(zs_server *s, int max_batch);
+/* out[4] = batches launched, requests served in them, graph replays of the
+ * batch launch, max_batch. */
+void zs_counters(zs_server *s, int64_t *out);
 /* Fails queued requests, waits for running ones, frees the pool.  No
  * zs_wait may still be blocked on this server. */
 void zs_destroy(zs_server *s);

@@ -105,6 +105,7 @@ def lib():
             "zo_sample": (None, [P, U64, U64, P]),
             "zo_chunk_sums": (None, [P, P]),
             "zo_run": (I64, [P, I32, U64, U64, P, P, P, I64, P]),
+            "zo_run_shots": (I32, [P, I32, U64, U64, P, I32, P]),
             "zo_refsim_apply": (None, [P, U32, P, I32, P]),
         }
         for name, (res, args) in sig.items():
@@ -393,6 +394,19 @@ def run(comp: Compiled, shots: int, seed: int, prec: int = 64, noise: dict | Non
     if m < 0:
         raise RuntimeError(lib().zo_error().decode())
     return {int(k): int(c) for k, c in zip(keys[:m], counts[:m])}, secs.value
+
+
+def run_shots(comp: Compiled, shots: int, seed: int, prec: int = 64, noise: dict | None = None,
+              threads: int = 1) -> np.ndarray:
+    """Per-shot clbit words of the per-shot path (src/exec.rs:459-528),
+    shot-parallel over `threads` workers with a state each."""
+    words = np.zeros(max(shots, 1), np.uint64)
+    nz = Noise(**{k: float(v) for k, v in noise.items()}) if noise else None
+    rc = lib().zo_run_shots(comp._h, prec, shots, seed, C.byref(nz) if nz is not None else None, threads,
+                            _ptr(words))
+    if rc < 0:
+        raise RuntimeError(lib().zo_error().decode())
+    return words[:shots]
 
 
 # --------------------------------------------------------------- RefSim

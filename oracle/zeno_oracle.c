@@ -1059,6 +1059,9 @@ int64_t zo_compiled_final_measures(const zo_compiled *c, uint32_t *pairs) {
 /* ============================================================ threading */
 
 static int g_threads = 1;
+/* Set on the shot-parallel workers of zo_run_shots: each owns a state, so
+ * its sweeps run on the calling thread (rayon nests them the same way). */
+static __thread int tl_inner_serial = 0;
 void zo_set_threads(int t) { g_threads = t < 1 ? 1 : t; }
 int zo_get_threads(void) { return g_threads; }
 
@@ -1088,7 +1091,7 @@ static void *pool_worker(void *arg) {
  * sweeps below are elementwise, so results do not depend on the split. */
 static void par_for(uint64_t total, task_fn fn, void *ctx) {
     if (total == 0) return;
-    uint64_t th = (uint64_t)g_threads;
+    uint64_t th = tl_inner_serial ? 1 : (uint64_t)g_threads;
     uint64_t nt = total <= 8192 ? 1 : total / 8192;
     if (nt > th * 8) nt = th * 8;
     if (nt < 1) nt = 1;
@@ -1751,19 +1754,108 @@ static double now_s(void) {
     return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
 }
 
-/* exec.rs:347-549: run_sampled for static circuits, run_per_shot (serial
- * shots) for dynamic or noisy ones.  Output: unique clbit words with
- * their counts, ascending by word. */
+/* exec.rs:485-500 run_shot: one trajectory from |0...0> with its own
+ * Xoshiro256++ stream seeded with splitmix64(seed ^ shot). */
+static uint64_t run_one_shot(zo_state *s, const zo_compiled *c, const zo_noise *nz,
+                             uint64_t seed, uint64_t shot) {
+    zo_reset_all(s);
+    rng_t r;
+    rng_seed(&r, zo_splitmix64(seed ^ shot));
+    uint64_t cl = 0;
+    for (int64_t i = 0; i < c->ops.n; i++) exec_shot_op(s, &c->ops.v[i], nz, &r, &cl);
+    for (int64_t f = 0; f < c->nfm; f++) {
+        int bit = zo_measure(s, c->fm[2 * f], rng_f64(&r));
+        if (nz) bit = readout(nz, bit, &r);
+        uint32_t cb = c->fm[2 * f + 1];
+        cl = (cl & ~(1ull << cb)) | ((uint64_t)bit << cb);
+    }
+    return cl;
+}
+
+typedef struct {
+    const zo_compiled *c;
+    const zo_noise *nz;
+    int prec;
+    uint64_t shots, seed, next; /* next: guarded by mu */
+    uint64_t *words;
+    int failed;
+    pthread_mutex_t mu;
+} shot_pool_t;
+
+static void *shot_worker(void *arg) {
+    shot_pool_t *p = (shot_pool_t *)arg;
+    tl_inner_serial = 1;
+    zo_state *s = zo_state_new(p->c->n_qubits, p->prec);
+    if (!s) {
+        pthread_mutex_lock(&p->mu);
+        p->failed = 1;
+        pthread_mutex_unlock(&p->mu);
+        return NULL;
+    }
+    for (;;) {
+        pthread_mutex_lock(&p->mu);
+        uint64_t shot = p->next++;
+        pthread_mutex_unlock(&p->mu);
+        if (shot >= p->shots) break;
+        p->words[shot] = run_one_shot(s, p->c, p->nz, p->seed, shot);
+    }
+    zo_state_free(s);
+    return NULL;
+}
+
+/* exec.rs:459-528 run_per_shot with its `parallel` fold: shots are dealt
+ * to `threads` workers, each with its own state (one backend per rayon
+ * worker); words[shot] gets shot's clbit word.  threads <= 1 runs the
+ * shots in order on one state.  Returns 0, or -1 when a state could not be
+ * allocated. */
+int zo_run_shots(const zo_compiled *c, int prec, uint64_t shots, uint64_t seed,
+                 const zo_noise *noise, int threads, uint64_t *words) {
+    const zo_noise *nz = noise_trivial(noise) ? NULL : noise;
+    if (shots == 0) return 0;
+    if (threads <= 1 || shots == 1) {
+        zo_state *s = zo_state_new(c->n_qubits, prec);
+        if (!s) return -1;
+        for (uint64_t shot = 0; shot < shots; shot++) words[shot] = run_one_shot(s, c, nz, seed, shot);
+        zo_state_free(s);
+        return 0;
+    }
+    shot_pool_t p;
+    p.c = c;
+    p.nz = nz;
+    p.prec = prec;
+    p.shots = shots;
+    p.seed = seed;
+    p.next = 0;
+    p.words = words;
+    p.failed = 0;
+    pthread_mutex_init(&p.mu, NULL);
+    uint64_t nw = (uint64_t)threads < shots ? (uint64_t)threads : shots;
+    pthread_t *ids = (pthread_t *)malloc(sizeof(pthread_t) * nw);
+    for (uint64_t i = 0; i < nw; i++) pthread_create(&ids[i], NULL, shot_worker, &p);
+    for (uint64_t i = 0; i < nw; i++) pthread_join(ids[i], NULL);
+    free(ids);
+    pthread_mutex_destroy(&p.mu);
+    if (p.failed) {
+        set_err("state allocation failed");
+        return -1;
+    }
+    return 0;
+}
+
+/* exec.rs:347-549: run_sampled for static circuits, run_per_shot for
+ * dynamic or noisy ones (shot-parallel over the oracle's thread count, as
+ * the reference folds shots over rayon's threads).  Output: unique clbit
+ * words with their counts, ascending by word. */
 int64_t zo_run(const zo_compiled *c, int prec, uint64_t shots, uint64_t seed,
                const zo_noise *noise, uint64_t *keys, uint64_t *counts,
                int64_t cap, double *sim_seconds) {
     const zo_noise *nz = noise_trivial(noise) ? NULL : noise;
     double t0 = now_s();
-    zo_state *s = zo_state_new(c->n_qubits, prec);
-    if (!s) return -1;
     kv *raw = NULL;
     uint64_t nraw = 0;
     if (!nz && !c->dynamic) {
+        zo_state *s = zo_state_new(c->n_qubits, prec);
+        if (!s) return -1;
         zo_apply_compiled(s, c, 0, c->ops.n);
         if (c->nfm > 0 && shots > 0) {
             uint64_t *smp = (uint64_t *)malloc(sizeof(uint64_t) * shots);
@@ -1781,27 +1873,21 @@ int64_t zo_run(const zo_compiled *c, int prec, uint64_t shots, uint64_t seed,
             }
             free(smp);
         }
+        zo_state_free(s);
     } else if (c->dynamic || c->nfm > 0) {
+        uint64_t *words = (uint64_t *)malloc(sizeof(uint64_t) * (shots ? shots : 1));
+        if (zo_run_shots(c, prec, shots, seed, nz, g_threads, words) != 0) {
+            free(words);
+            return -1;
+        }
         raw = (kv *)malloc(sizeof(kv) * (shots ? shots : 1));
         for (uint64_t shot = 0; shot < shots; shot++) {
-            zo_reset_all(s);
-            rng_t r;
-            rng_seed(&r, zo_splitmix64(seed ^ shot));
-            uint64_t cl = 0;
-            for (int64_t i = 0; i < c->ops.n; i++)
-                exec_shot_op(s, &c->ops.v[i], nz, &r, &cl);
-            for (int64_t f = 0; f < c->nfm; f++) {
-                int bit = zo_measure(s, c->fm[2 * f], rng_f64(&r));
-                if (nz) bit = readout(nz, bit, &r);
-                uint32_t cb = c->fm[2 * f + 1];
-                cl = (cl & ~(1ull << cb)) | ((uint64_t)bit << cb);
-            }
-            raw[nraw].key = cl;
+            raw[nraw].key = words[shot];
             raw[nraw++].n = 1;
         }
+        free(words);
     }
     if (sim_seconds) *sim_seconds = now_s() - t0;
-    zo_state_free(s);
     qsort(raw, nraw, sizeof(kv), cmp_kv);
     int64_t m = 0;
     for (uint64_t i = 0; i < nraw; i++) {

@@ -126,6 +126,12 @@ int64_t zo_run(const zo_compiled *c, int prec, uint64_t shots, uint64_t seed,
                const zo_noise *noise, uint64_t *keys, uint64_t *counts,
                int64_t cap, double *sim_seconds);
 
+/* exec.rs:459-528 run_per_shot with its shot-parallel fold: words[shot]
+ * gets each shot's clbit word; shots are dealt to `threads` workers with a
+ * state each (threads <= 1: in order on one state).  0 or -1. */
+int zo_run_shots(const zo_compiled *c, int prec, uint64_t shots, uint64_t seed,
+                 const zo_noise *noise, int threads, uint64_t *words);
+
 /* ---- RefSim (tests/reference.rs:69-116) ---------------------------- */
 void zo_refsim_apply(double *amps, uint32_t n, const uint32_t *qs, int k,
                      const double *mat);

@@ -98,6 +98,35 @@ def test_format_key_and_mem_planning():
     z.plan(20, 64, 1 << 30)
 
 
+def test_plan_and_resolve_precision_through_the_c_abi(monkeypatch):
+    """mem::plan / resolve_precision (src/mem.rs:114-130, src/exec.rs:261-304)
+    behind zr_plan / zr_resolve_precision: Error::Memory's fields and text,
+    the auto-precision fallback and its notice, and mem::budget_bytes'
+    precedence (explicit limit, then ZENO_MEM_BYTES x fraction)."""
+    with pytest.raises(z.MemoryBudget) as ei:
+        z.plan(30, 64, 1 << 30)
+    e = ei.value
+    assert (e.n_qubits, e.precision, e.needed_bytes, e.budget_bytes, e.max_qubits) == (30, 64, 16 << 30, 1 << 30, 26)
+    assert "needs 16.0 GiB but the budget is 1.0 GiB (max 26 qubits at this precision" in str(e)
+    assert z.plan(26, 64, 1 << 30) == 1 << 30  # exactly at the budget fits (needed <= budget)
+    # auto: f64 when it fits, else f32 with the notice; an explicit precision never falls back
+    assert z.resolve_precision(29, z.RunOptions(mem_limit=8 << 30)) == (64, [])
+    assert z.resolve_precision(30, z.RunOptions(mem_limit=(16 << 30) - 1)) == (
+        32, ["auto-precision: 30 qubits exceed the f64 budget, using f32 (8 B/amplitude)"])
+    with pytest.raises(z.MemoryBudget):
+        z.resolve_precision(30, z.RunOptions(mem_limit=(16 << 30) - 1, precision=64))
+    with pytest.raises(z.MemoryBudget) as ei:
+        z.resolve_precision(31, z.RunOptions(mem_limit=(16 << 30) - 1))
+    assert ei.value.precision == 32 and ei.value.max_qubits == 30
+    # budget precedence: explicit limit wins; otherwise ZENO_MEM_BYTES * fraction
+    monkeypatch.setenv("ZENO_MEM_BYTES", str(40 << 30))
+    assert z.budget_bytes(123, 0.75) == 123
+    assert z.budget_bytes(None, 0.75) == 30 << 30
+    assert z.resolve_precision(31, z.RunOptions(mem_fraction=0.8)) == (64, [])  # 32 GiB: exactly fits
+    assert z.resolve_precision(31, z.RunOptions()) == (32, [
+        "auto-precision: 31 qubits exceed the f64 budget, using f32 (8 B/amplitude)"])
+
+
 def test_teleport_compiles_dynamic():
     I = z.ir.instr
     rows = [I("x", [0]), I("h", [1]), I("cx", [1, 2]), I("cx", [0, 1]), I("h", [0]),

@@ -431,3 +431,48 @@ def test_noise_determinism_and_trivial_model(orc):
     ideal, _ = orc.run(comp, 300, 42)
     triv, _ = orc.run(comp, 300, 42, noise={k: 0.0 for k in nz})
     assert ideal == triv
+
+
+def test_shot_parallel_per_shot_path_equals_serial(orc):
+    """run_per_shot's shot-parallel fold (src/exec.rs:507-528): dealing the
+    shots to workers with a state each gives every shot the word the serial
+    loop gives it, and zo_run's counts at any thread count are the same."""
+    nz = {"depolarizing_1q": 0.02, "depolarizing_2q": 0.05, "bit_flip": 0.01, "phase_flip": 0.01,
+          "amplitude_damping": 0.03, "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.03}
+    I = orc.instr
+    prog = orc.program([orc.gen_random(8, 5, 12), orc.reset(2), orc.measure(4, 0),
+                        I("ry", [1], [0.4], cond=(0, 1)), orc.measure_all(8)])
+    c = orc.Compiled(prog, 8, cregs=[8], fusion=0)
+    for prec in (64, 32):
+        serial = orc.run_shots(c, 300, 31, prec, nz, threads=1)
+        for t in (2, 5, 8):
+            assert np.array_equal(orc.run_shots(c, 300, 31, prec, nz, threads=t), serial)
+        v, k = np.unique(serial, return_counts=True)
+        want = {int(a): int(b) for a, b in zip(v, k)}
+        orc.set_threads(6)
+        try:
+            assert orc.run(c, 300, 31, prec=prec, noise=nz)[0] == want
+        finally:
+            orc.set_threads(1)
+        # shot s of a run seeded S is the single shot of a run seeded S ^ s
+        assert int(orc.run_shots(c, 1, 31 ^ 17, prec, nz)[0]) == int(serial[17])
+
+
+def test_trajectory_fixtures_are_the_oracles(orc):
+    """tests/golden/trajectories_16q_d30_f{32,64}.npy hold the oracle's
+    per-shot words for the noisy-trajectory configuration: recompute a
+    spread of shots (each shot s as the single shot of seed 99 ^ s)."""
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_fixtures as mf
+
+    c = mf.compiled()
+    for prec in (32, 64):
+        words = np.load(os.path.join(os.path.dirname(mf.__file__), f"trajectories_16q_d30_f{prec}.npy"))
+        assert words.dtype == np.uint64 and len(words) == mf.SHOTS
+        for s in (0, 1, 777, 2048, 4095):
+            assert int(orc.run_shots(c, 1, mf.SEED ^ s, prec, mf.NOISE)[0]) == int(words[s]), (prec, s)
+        # the readout flips make most words distinct; a broken writer would not
+        assert len(np.unique(words)) > 1000

@@ -25,10 +25,14 @@ from .exec import (
     RunOptions,
     RunResult,
     Simulator,
+    batch_supported,
+    budget_bytes,
     compile,
     format_key,
     max_qubits,
     plan,
+    resolve_precision,
+    run_batch,
     run_compiled,
     run_program,
     run_trajectories,
@@ -42,6 +46,6 @@ from . import qasm, tune
 __all__ = [
     "Circuit", "Compiled", "calibration_copy", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
     "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
-    "format_key", "ir", "qasm", "tune", "matching_circuit", "max_qubits", "plan", "qft", "random_circuit",
+    "format_key", "ir", "qasm", "tune", "matching_circuit", "max_qubits", "plan", "budget_bytes", "resolve_precision", "batch_supported", "run_batch", "qft", "random_circuit",
     "Served", "Server", "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",
 ]

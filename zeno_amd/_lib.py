@@ -56,6 +56,9 @@ SIGNATURES = {
     "zr_compiled_final_measures": (I64, [P, P]),
     "zr_apply_ops": (I32, [P, P, I64, I64]),
     "zr_run": (I64, [P, I32, U64, U64, P, I32, P, P, I64, P]),
+    "zr_budget_bytes": (U64, [U64, D]),
+    "zr_plan": (I32, [U32, I32, U64, P]),
+    "zr_resolve_precision": (I32, [U32, I32, U64, P, P, C.c_char_p, C.c_size_t]),
     "zb_device_count": (I32, []),
     "zb_mem_info": (I32, [P, P]),
     "zb_calibration_copy": (I32, [U64, I32, P]),
@@ -66,9 +69,13 @@ SIGNATURES = {
     "zs_submit": (I64, [P, P, I32, U64, U64, P]),
     "zs_wait": (I64, [P, I64, P, P, I64, P]),
     "zs_status": (None, [P, P]),
+    "zs_set_batch": (I32, [P, I32]),
+    "zs_counters": (None, [P, P]),
     "zs_destroy": (None, [P]),
     "zr_parse_qasm": (I64, [C.c_char_p, P, I64, P, P, P, U32]),
     "zr_trajectories_supported": (I32, [P]),
+    "zr_batch_supported": (I32, [P, I32, U64]),
+    "zr_run_batch": (I32, [P, P, P, P, I64, P, P]),
     "zr_run_trajectories": (I32, [P, I32, U64, U64, P, P, P]),
     "zr_gen_random": (I64, [U32, U32, U64, P, I64]),
     "zr_gen_qft": (I64, [U32, P, I64]),

@@ -51,6 +51,7 @@ unsigned grid_for(uint64_t work) {
 }  // namespace
 
 struct zb_backend {
+    int device = 0;  // the HIP device current when the handle was created
     uint32_t n = 0;
     int prec = 64;
     uint64_t len = 0;
@@ -940,14 +941,36 @@ void chunk_sums_T(zb_backend *h, double *dsums, uint64_t chunk, uint64_t nch) {
 constexpr uint64_t kDeviceSortShots = 1ull << 15;
 constexpr uint64_t kDeviceSortMaxShots = 1ull << 30;  // the radix sort counts items in an int
 
+// Device scratch the sampler needs for `shots` draws from a state of `len`
+// amplitudes: the largest of its three layouts (chunk sums; the device sort
+// and routing; the host-routed rescan with at most one touched entry per
+// chunk), each as laid out below.
+size_t sample_scratch_bytes(uint64_t len, uint64_t shots) {
+    const uint64_t chunk = std::min<uint64_t>(len, 1ull << 16);
+    const uint64_t nch = (len + chunk - 1) / chunk;
+    size_t need = nch * sizeof(double);
+    if (shots >= kDeviceSortShots && shots <= kDeviceSortMaxShots) {
+        const size_t tb = (draw_sort_temp_bytes(shots) + 255) & ~size_t(255);
+        const size_t nb = ((nch + 1) * 8 + 255) & ~size_t(255), sb = (shots * 8 + 255) & ~size_t(255);
+        need = std::max(need, tb + 2 * nb + 3 * sb + 256);
+    } else {
+        need = std::max<size_t>(need, std::min(nch, shots) * 4 + (nch + 1) * 8 * 2 + shots * 8 * 2 + 64);
+    }
+    return need;
+}
+
+// The sampler's device scratch comes from the handle's grow-only buffer, or,
+// when `fixed` is given, from that region (which the caller has checked
+// holds sample_scratch_bytes); nothing is then allocated.
 template <typename T>
-void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
+void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out, char *fixed = nullptr) {
     flush(h);
     refresh_layout(h);
     if (shots == 0) return;
+    auto get_scratch = [&](size_t bytes) { return fixed ? fixed : h->scratch_get(bytes); };
     const uint64_t chunk = std::min<uint64_t>(h->len, 1ull << 16);
     const uint64_t nch = (h->len + chunk - 1) / chunk;
-    double *dsums = (double *)h->scratch_get(nch * sizeof(double));
+    double *dsums = (double *)get_scratch(nch * sizeof(double));
     chunk_sums_T<T>(h, dsums, chunk, nch);
     std::vector<double> sums(nch);
     check(hipMemcpyAsync(sums.data(), dsums, nch * sizeof(double), hipMemcpyDeviceToHost, h->stream), "sums copy");
@@ -962,7 +985,7 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
         // sort, route and rescan on the device; every chunk gets a block
         const size_t tb = (draw_sort_temp_bytes(shots) + 255) & ~size_t(255);
         const size_t nb = ((nch + 1) * 8 + 255) & ~size_t(255), sb = (shots * 8 + 255) & ~size_t(255);
-        char *p = h->scratch_get(tb + 2 * nb + 3 * sb + 256);
+        char *p = get_scratch(tb + 2 * nb + 3 * sb + 256);
         p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
         void *temp = p;
         double *dcum = (double *)(p + tb);
@@ -987,7 +1010,7 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
         if (starts[c] < starts[c + 1]) touched.push_back((uint32_t)c);
     if (!touched.empty()) {
         const size_t bytes = touched.size() * 4 + (nch + 1) * 8 * 2 + shots * 8 * 2;
-        char *p = h->scratch_get(bytes + 64);
+        char *p = get_scratch(bytes + 64);
         double *dcum = (double *)p;
         p += (nch + 1) * 8;
         uint64_t *dstarts = (uint64_t *)p;
@@ -1166,6 +1189,21 @@ int check_shape(uint32_t n_qubits, int precision) {
     return ZB_OK;
 }
 
+// Every tuning knob back to the library default (a fresh handle's value),
+// so a re-bound handle carries nothing over from the shape it ran before.
+void reset_knobs(zb_backend *h) {
+    const zb_backend d;
+    h->prefetch = d.prefetch;
+    h->slots = d.slots;
+    h->win_grid = d.win_grid;
+    h->page_bits = d.page_bits;
+    h->tile_bits = d.tile_bits;
+    h->defer_cx = d.defer_cx;
+    h->relabel = d.relabel;
+    h->mfma = d.mfma;
+    h->fma = d.fma;
+}
+
 void set_shape(zb_backend *h, uint32_t n_qubits, int precision) {
     h->n = n_qubits;
     h->prec = precision;
@@ -1174,8 +1212,12 @@ void set_shape(zb_backend *h, uint32_t n_qubits, int precision) {
     h->name = precision == 64 ? "hip-f64" : "hip-f32";
 }
 
-// Stream, parameter arena and pinned result slot: everything but the state.
+// Stream, parameter arena, pinned result slot and layout table: everything
+// but the state (the layout table too, so that no read of the state has to
+// allocate later).
 void create_resources(zb_backend *h) {
+    check(hipGetDevice(&h->device), "hipGetDevice");
+    check(hipMalloc((void **)&h->lut_dev, 5 * 256 * sizeof(uint64_t)), "hipMalloc layout");
     check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipMalloc((void **)&h->arena_dev, kArenaBytes), "hipMalloc arena");
     check(hipHostMalloc((void **)&h->arena_host, kArenaBytes, hipHostMallocDefault), "hipHostMalloc arena");
@@ -1282,6 +1324,7 @@ int zb_rebind(zb_backend *h, uint32_t n_qubits, int precision, void *re, void *i
         h->re = re;
         h->im = im;
         set_shape(h, n_qubits, precision);
+        reset_knobs(h);
         set_layout(h, (int)default_layout(h));
         apply_defaults(h);
         zero_state(h);

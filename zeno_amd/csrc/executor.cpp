@@ -5,9 +5,12 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <stdexcept>
+#include <string>
 #include <unordered_map>
 
 #include "host.hpp"
@@ -171,7 +174,8 @@ bool valid(const zr_noise *m) {  // noise.rs:78-87
 namespace zeno {
 
 int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint64_t seed, const zr_noise *noise,
-                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out) {
+                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out,
+                    TrajWorkspace *traj) {
     const Compiled &c = cc->c;
     if (!valid(noise)) {
         set_error("invalid noise model: every probability must be in [0, 1]");
@@ -205,23 +209,33 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
                     raw[cl] += kv.second;
                 }
             }
-        } else if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && zr_trajectories_supported(cc)) {
-            // run_per_shot (exec.rs:459-549) as one batched trajectory launch
-            std::vector<uint64_t> words(shots);
-            ok(zr_run_trajectories(cc, precision, shots, seed, nz, words.data(), nullptr));
-            for (uint64_t w : words) ++raw[w];
-        } else if (c.dynamic || !c.final_measures.empty()) {  // run_per_shot (exec.rs:459-549)
-            for (uint64_t shot = 0; shot < shots; ++shot) {
-                ok(zb_reset_all(h));
-                Rng rng(splitmix64(seed ^ shot));
-                uint64_t cl = 0;
-                for (const Op &o : c.ops) exec_shot_op(h, o, nz, rng, cl);
-                for (const auto &fm : c.final_measures) {
-                    int bit = measure(h, fm.first, rng.f64());
-                    if (nz) bit = readout(*nz, bit, rng);
-                    cl = (cl & ~(1ull << fm.second)) | ((uint64_t)bit << fm.second);
+        } else {  // run_per_shot (exec.rs:459-549)
+            bool batched = false;
+            if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && trajectories_supported(c)) {
+                // every shot in one batched trajectory launch; when no
+                // workspace fits, the shots run one by one below
+                std::vector<uint64_t> words(shots);
+                const int rc = traj ? run_trajectories_in(c, precision, shots, seed, nz, words.data(), nullptr, *traj)
+                                    : zr_run_trajectories(cc, precision, shots, seed, nz, words.data(), nullptr);
+                if (rc != ZB_ERR_MEMORY) ok(rc);
+                if (rc == ZB_OK) {
+                    batched = true;
+                    for (uint64_t w : words) ++raw[w];
                 }
-                ++raw[cl];
+            }
+            if (!batched && (c.dynamic || !c.final_measures.empty())) {
+                for (uint64_t shot = 0; shot < shots; ++shot) {
+                    ok(zb_reset_all(h));
+                    Rng rng(splitmix64(seed ^ shot));
+                    uint64_t cl = 0;
+                    for (const Op &o : c.ops) exec_shot_op(h, o, nz, rng, cl);
+                    for (const auto &fm : c.final_measures) {
+                        int bit = measure(h, fm.first, rng.f64());
+                        if (nz) bit = readout(*nz, bit, rng);
+                        cl = (cl & ~(1ull << fm.second)) | ((uint64_t)bit << fm.second);
+                    }
+                    ++raw[cl];
+                }
             }
         }
     } catch (const RunError &e) {
@@ -294,6 +308,100 @@ int zr_apply_ops(zb_backend *h, const zr_compiled *c, int64_t lo, int64_t hi) {
         return e.code;
     }
     return ZB_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+// mem::state_bytes (src/mem.rs:79-81), saturated at 2^64 - 1.
+uint64_t state_bytes(uint32_t n, int precision) {
+    const unsigned __int128 b = ((unsigned __int128)1 << n) * (unsigned __int128)(precision == 64 ? 16 : 8);
+    return b > (unsigned __int128)UINT64_MAX ? UINT64_MAX : (uint64_t)b;
+}
+
+// mem::max_qubits (src/mem.rs:84-93).
+uint32_t max_qubits(uint64_t budget, int precision) {
+    uint32_t n = 0;
+    while (state_bytes(n + 1, precision) <= budget) {
+        ++n;
+        if (n >= 60) break;
+    }
+    return n;
+}
+
+// lib.rs human_bytes: B below 1 KiB, else one decimal and the largest unit.
+std::string human_bytes(uint64_t b) {
+    static const char *units[] = {"B", "KiB", "MiB", "GiB", "TiB"};
+    double v = (double)b;
+    int u = 0;
+    while (v >= 1024.0 && u + 1 < 5) v /= 1024.0, ++u;
+    char buf[64];
+    if (u == 0) std::snprintf(buf, sizeof buf, "%llu B", (unsigned long long)b);
+    else std::snprintf(buf, sizeof buf, "%.1f %s", v, units[u]);
+    return buf;
+}
+
+}  // namespace
+
+extern "C" {
+
+uint64_t zr_budget_bytes(uint64_t mem_limit, double fraction) {
+    if (mem_limit > 0) return mem_limit;
+    uint64_t physical = 0;
+    if (const char *env = std::getenv("ZENO_MEM_BYTES")) {
+        char *end = nullptr;
+        const unsigned long long v = std::strtoull(env, &end, 10);
+        if (end && end != env && *end == '\0') physical = v;
+    }
+    if (physical == 0) {
+        uint64_t free_b = 0, total_b = 0;
+        if (zb_device_count() == 0 || zb_mem_info(&free_b, &total_b) != ZB_OK) return 0;
+        physical = total_b;
+    }
+    return (uint64_t)((double)physical * fraction);
+}
+
+int zr_plan(uint32_t n_qubits, int precision, uint64_t budget, zr_mem_plan *plan) {
+    if (precision != 32 && precision != 64) {
+        set_error("precision must be 32 or 64");
+        return ZB_ERR_ARG;
+    }
+    zr_mem_plan p{};
+    p.n_qubits = n_qubits;
+    p.precision = precision;
+    p.needed_bytes = state_bytes(n_qubits, precision);
+    p.budget_bytes = budget;
+    p.max_qubits = max_qubits(budget, precision);
+    if (plan) *plan = p;
+    if (p.needed_bytes > budget) {  // Error::Memory's text (src/lib.rs:82-88)
+        set_error("state vector for " + std::to_string(n_qubits) + " qubits at f" + std::to_string(precision) +
+                  " needs " + human_bytes(p.needed_bytes) + " but the budget is " + human_bytes(budget) + " (max " +
+                  std::to_string(p.max_qubits) +
+                  " qubits at this precision; raise --mem-limit or lower qubits)");
+        return ZB_ERR_MEMORY;
+    }
+    return ZB_OK;
+}
+
+int zr_resolve_precision(uint32_t n_qubits, int requested, uint64_t budget, int *precision, zr_mem_plan *plan,
+                         char *notice, size_t notice_cap) {
+    if (notice && notice_cap) notice[0] = '\0';
+    if (requested != 0 && requested != 32 && requested != 64) {
+        set_error("requested precision must be 0 (auto), 32 or 64");
+        return ZB_ERR_ARG;
+    }
+    int chosen = requested;
+    if (requested == 0) {
+        chosen = zr_plan(n_qubits, 64, budget, nullptr) == ZB_OK ? 64 : 32;
+        if (chosen == 32 && zr_plan(n_qubits, 32, budget, nullptr) == ZB_OK && notice && notice_cap) {
+            const std::string t = "auto-precision: " + std::to_string(n_qubits) +
+                                  " qubits exceed the f64 budget, using f32 (8 B/amplitude)";
+            std::snprintf(notice, notice_cap, "%s", t.c_str());
+        }
+    }
+    if (precision) *precision = chosen;
+    return zr_plan(n_qubits, chosen, budget, plan);
 }
 
 int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t seed, const zr_noise *noise, int mode,

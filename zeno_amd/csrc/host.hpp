@@ -91,13 +91,18 @@ struct zr_compiled {
 
 namespace zeno {
 
+struct TrajWorkspace;
+
 // exec::run_compiled (src/exec.rs:407-549) on an existing backend whose
 // qubit count matches: resets it to |0...0> (unless the caller says it
-// already is), runs the circuit (sampled,
-// batched trajectories, or per shot) and writes the clbit-word counts,
-// ascending by word.  Returns ZB_OK or a negative ZB_ERR_*.
+// already is), runs the circuit (sampled, batched trajectories, or per
+// shot) and writes the clbit-word counts, ascending by word.  Batched
+// trajectories run in `traj` when given (no device allocation), else in a
+// workspace allocated for the call; when that does not fit, the shots run
+// one by one on h.  Returns ZB_OK or a negative ZB_ERR_*.
 int run_compiled_on(zb_backend *h, const zr_compiled *c, uint64_t shots, uint64_t seed, const zr_noise *noise,
-                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out);
+                    int mode, bool state_is_zero, std::vector<std::pair<uint64_t, uint64_t>> &out,
+                    TrajWorkspace *traj = nullptr);
 
 // Window scheduling hint: need[i] is the mask of circuit qubits that the
 // i-th coming op needs inside a tile (a dense op's qubits, a CX's target),
@@ -109,6 +114,61 @@ void backend_lookahead_pos(zb_backend *h, size_t pos);
 // 0..low-1 onto the low state bits; call while the hint is still set.
 // Returns a ZB_* code.
 int backend_end_run(zb_backend *h);
+
+// Batched trajectories (trajectories.hip) in memory the caller provides:
+// `mem` holds the states of the concurrent trajectories plus the op tables
+// and per-shot words (traj_workspace_bytes); `stream` is the launch stream.
+// With a fixed `launch_block` (device memory of at least 256 bytes that the
+// holder keeps), the captured launch in `graph` is replayed by every later
+// run at the same grid and precision; the holder frees it with
+// traj_graph_free.  No device memory is allocated.
+struct TrajGraph;
+struct TrajWorkspace {
+    void *mem = nullptr;
+    uint64_t bytes = 0;
+    void *stream = nullptr;        // hipStream_t
+    void *launch_block = nullptr;  // optional, see above
+    TrajGraph *graph = nullptr;
+};
+constexpr uint64_t kTrajLaunchBlockBytes = 256;
+// Concurrent trajectories worth running: min(shots, 2 per CU), at least 1.
+uint64_t traj_max_grid(uint64_t shots);
+// Workspace bytes for `grid` concurrent trajectories of c.
+uint64_t traj_workspace_bytes(const Compiled &c, int precision, uint64_t shots, uint64_t grid);
+// The grid a workspace of `bytes` holds (0: not even one state).
+uint64_t traj_grid_for(const Compiled &c, int precision, uint64_t shots, uint64_t bytes);
+int run_trajectories_in(const Compiled &c, int precision, uint64_t shots, uint64_t seed, const zr_noise *noise,
+                        uint64_t *out, double *secs, TrajWorkspace &w);
+void traj_graph_free(TrajGraph *g);
+uint64_t traj_graph_replays(const TrajGraph *g);
+bool trajectories_supported(const Compiled &c);
+
+// Small static circuits as instances of one persistent launch (batch.hip).
+// A runner owns a stream, a device workspace with its pinned staging, a
+// fixed launch block and the graphs it captured (one per precision, state
+// size and grid, replayed whenever a batch of that shape comes).
+constexpr uint64_t kBatchMaxShots = 1ull << 17;
+struct BatchRunner;
+struct BatchItem {
+    const Compiled *c = nullptr;
+    int precision = 64;
+    uint64_t shots = 0, seed = 0;   // the run's seed, as zr_run takes it
+    std::vector<uint64_t> indices;  // out: sampled basis indices, in draw order
+};
+// Static, n <= 13 (f64) / 14 (f32), dense ops on <= 3 (f64) / 4 (f32)
+// qubits, <= 2^17 shots.
+bool batch_supported(const Compiled &c, int precision, uint64_t shots);
+// Workspace bytes one instance takes in a batch.
+uint64_t batch_item_bytes(const Compiled &c, int precision, uint64_t shots);
+BatchRunner *batch_runner_new(uint64_t workspace_bytes);
+void batch_runner_free(BatchRunner *r);
+uint64_t batch_runner_capacity(const BatchRunner *r);
+uint64_t batch_runner_replays(const BatchRunner *r);
+// Runs the items (one launch per precision present); ZB_ERR_MEMORY when
+// they do not fit the runner's workspace together.
+int batch_run(BatchRunner *r, std::vector<BatchItem> &items);
+// Sampled indices -> clbit words through the final measures.
+void batch_words(const Compiled &c, const std::vector<uint64_t> &idx, uint64_t *words);
 
 // Thread-local last-error plumbing for the C ABI.
 void set_error(const std::string &msg);

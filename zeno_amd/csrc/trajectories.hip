@@ -26,18 +26,6 @@ void hcheck(hipError_t e, const char *what) {
     if (e != hipSuccess) throw TrajError{ZB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
 }
 
-// Device buffers freed on every exit path.
-struct DevBuf {
-    void *p = nullptr;
-    ~DevBuf() {
-        if (p) (void)hipFree(p);
-    }
-    template <typename U>
-    U *get() {
-        return static_cast<U *>(p);
-    }
-};
-
 bool supported(const Compiled &c) {
     if (c.n_qubits > 20) return false;
     for (const Op &o : c.ops) {
@@ -46,112 +34,221 @@ bool supported(const Compiled &c) {
     return true;
 }
 
+constexpr uint64_t kAlign = 256;
+uint64_t up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
+
+// The op tables of one run in the kernel's format: ops, matrices in T,
+// final-measure pairs.
 template <typename T>
-void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noise, uint64_t *out, double *secs) {
+struct Tables {
     std::vector<dev::TrajOp> ops;
     std::vector<T> mats;
+    std::vector<uint32_t> fm;
+    size_t nops = 0, nfm = 0;
+};
+
+template <typename T>
+Tables<T> pack(const Compiled &c) {
+    Tables<T> t;
     for (const Op &o : c.ops) {
         if (o.kind == kBarrier) continue;
-        dev::TrajOp t{};
-        t.is_if = o.is_if;
-        t.creg_off = o.creg_off;
-        t.creg_len = o.creg_len;
-        t.value = o.value;
+        dev::TrajOp d{};
+        d.is_if = o.is_if;
+        d.creg_off = o.creg_off;
+        d.creg_len = o.creg_len;
+        d.value = o.value;
         if (o.kind == kUnitary || o.kind == kDiagonal) {
-            t.kind = o.kind == kUnitary ? 0 : 1;
-            t.k = (uint32_t)o.qs.size();
-            for (size_t b = 0; b < o.qs.size(); ++b) t.q[b] = o.qs[b];
-            t.data_off = (uint32_t)mats.size();
-            for (const cplx &z : o.data) mats.push_back((T)z.re);  // T::fr
-            for (const cplx &z : o.data) mats.push_back((T)z.im);
+            d.kind = o.kind == kUnitary ? 0 : 1;
+            d.k = (uint32_t)o.qs.size();
+            for (size_t b = 0; b < o.qs.size(); ++b) d.q[b] = o.qs[b];
+            d.data_off = (uint32_t)t.mats.size();
+            for (const cplx &z : o.data) t.mats.push_back((T)z.re);  // T::fr
+            for (const cplx &z : o.data) t.mats.push_back((T)z.im);
         } else if (o.kind == kCx) {
-            t.kind = 2;
-            t.q[0] = o.a, t.q[1] = o.b;
+            d.kind = 2;
+            d.q[0] = o.a, d.q[1] = o.b;
         } else if (o.kind == kMeasure) {
-            t.kind = 3;
-            t.q[0] = o.a;
-            t.clbit = o.b;
+            d.kind = 3;
+            d.q[0] = o.a;
+            d.clbit = o.b;
         } else if (o.kind == kReset) {
-            t.kind = 4;
-            t.q[0] = o.a;
+            d.kind = 4;
+            d.q[0] = o.a;
         }
-        ops.push_back(t);
+        t.ops.push_back(d);
     }
-    std::vector<uint32_t> fm;
-    for (const auto &f : c.final_measures) fm.push_back(f.first), fm.push_back(f.second);
-    const size_t nops = ops.size(), nfm = c.final_measures.size();
-    // keep every upload non-empty; the kernel reads only nops / nfm entries
-    if (mats.empty()) mats.push_back(T(0));
-    if (ops.empty()) ops.push_back(dev::TrajOp{});
-    if (fm.empty()) fm.push_back(0);
+    for (const auto &f : c.final_measures) t.fm.push_back(f.first), t.fm.push_back(f.second);
+    t.nops = t.ops.size();
+    t.nfm = c.final_measures.size();
+    // keep every region non-empty; the kernel reads only nops / nfm entries
+    if (t.mats.empty()) t.mats.push_back(T(0));
+    if (t.ops.empty()) t.ops.push_back(dev::TrajOp{});
+    if (t.fm.empty()) t.fm.push_back(0);
+    return t;
+}
 
-    int dev_id = 0, cus = 0;
-    hcheck(hipGetDevice(&dev_id), "hipGetDevice");
-    hcheck(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev_id), "cu count");
-    const uint64_t len = 1ull << c.n_qubits;
-    const uint64_t grid = std::max<uint64_t>(1, std::min<uint64_t>(shots, (uint64_t)cus * 2));
-    size_t free_b = 0, total_b = 0;
-    hcheck(hipMemGetInfo(&free_b, &total_b), "hipMemGetInfo");
-    const size_t ws_bytes = (size_t)grid * 2 * len * sizeof(T);
-    if (ws_bytes + shots * 8 + (64ull << 20) > free_b)
-        throw TrajError{ZB_ERR_MEMORY, "trajectory workspace of " + std::to_string(ws_bytes) + " B does not fit"};
-    DevBuf ws, dops, dmats, dfm, dout;
-    hcheck(hipMalloc(&ws.p, ws_bytes), "hipMalloc workspace");
-    hcheck(hipMalloc(&dops.p, ops.size() * sizeof(dev::TrajOp)), "hipMalloc ops");
-    hcheck(hipMalloc(&dmats.p, mats.size() * sizeof(T)), "hipMalloc mats");
-    hcheck(hipMalloc(&dfm.p, fm.size() * 4), "hipMalloc fm");
-    hcheck(hipMalloc(&dout.p, std::max<uint64_t>(shots, 1) * 8), "hipMalloc out");
-    dev::TrajNoise nz{};
-    int noisy = 0;
-    if (noise) {
-        nz = dev::TrajNoise{noise->depolarizing_1q, noise->depolarizing_2q, noise->bit_flip, noise->phase_flip,
-                            noise->amplitude_damping, noise->readout_flip_0to1, noise->readout_flip_1to0};
-        noisy = 1;
+// Workspace layout: [grid states][launch block][ops][mats][fm][per-shot words].
+template <typename T>
+uint64_t tables_bytes(const Compiled &c, uint64_t shots) {
+    uint64_t nops = 1, nmat = 1, nfm = 2 * c.final_measures.size() + 1;
+    for (const Op &o : c.ops) {
+        if (o.kind == kBarrier) continue;
+        ++nops;
+        if (o.kind == kUnitary || o.kind == kDiagonal) nmat += 2 * o.data.size();
     }
-    hipStream_t stream;
-    hcheck(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    const auto t0 = std::chrono::steady_clock::now();
-    try {
-        // Uploads on the launch stream, waited for: a plain hipMemcpy from
-        // pageable memory may return before its DMA lands, and a
-        // non-blocking stream would not wait for it.
-        hcheck(hipMemcpyAsync(dops.p, ops.data(), ops.size() * sizeof(dev::TrajOp), hipMemcpyHostToDevice, stream),
-               "ops");
-        hcheck(hipMemcpyAsync(dmats.p, mats.data(), mats.size() * sizeof(T), hipMemcpyHostToDevice, stream), "mats");
-        hcheck(hipMemcpyAsync(dfm.p, fm.data(), fm.size() * 4, hipMemcpyHostToDevice, stream), "fm");
-        hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-        // One persistent launch, captured in a graph and replayed once.
-        hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        hipLaunchKernelGGL(dev::k_trajectories<T>, dim3((unsigned)grid), dim3(dev::kBlock), 0, stream, ws.get<T>(),
-                           c.n_qubits, dops.get<dev::TrajOp>(), (uint32_t)nops, dfm.get<uint32_t>(), (uint32_t)nfm,
-                           dmats.get<T>(), nz, noisy, seed, shots, dout.get<uint64_t>());
-        hcheck(hipGetLastError(), "k_trajectories");
-        hcheck(hipStreamEndCapture(stream, &graph), "hipStreamEndCapture");
-        hcheck(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
-        hcheck(hipGraphLaunch(exec, stream), "hipGraphLaunch");
-        hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
-    } catch (...) {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        (void)hipStreamDestroy(stream);
-        throw;
-    }
-    if (secs) *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    (void)hipGraphExecDestroy(exec);
-    (void)hipGraphDestroy(graph);
-    (void)hipStreamDestroy(stream);
-    if (shots) hcheck(hipMemcpy(out, dout.p, shots * 8, hipMemcpyDeviceToHost), "out copy");
+    return up(sizeof(dev::TrajLaunch<T>)) + up(nops * sizeof(dev::TrajOp)) + up(nmat * sizeof(T)) + up(nfm * 4) +
+           up(std::max<uint64_t>(shots, 1) * 8);
+}
+
+uint64_t state_bytes_of(const Compiled &c, int precision) {
+    return up(2 * (1ull << c.n_qubits) * (precision == 64 ? 8 : 4));
+}
+
+uint64_t cu_count() {
+    static const uint64_t cus = [] {
+        int d = 0, n = 0;
+        if (hipGetDevice(&d) != hipSuccess || hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, d) !=
+                                                   hipSuccess || n <= 0)
+            n = 256;
+        return (uint64_t)n;
+    }();
+    return cus;
 }
 
 }  // namespace
 
-extern "C" int zr_trajectories_supported(const zr_compiled *c) { return supported(c->c) ? 1 : 0; }
+namespace zeno {
 
-extern "C" int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots, uint64_t seed,
-                                   const zr_noise *noise, uint64_t *out, double *kernel_seconds) {
-    if (!supported(c->c)) {
+// A captured launch: replayed as long as the launch block's address, the
+// grid and the precision repeat (the block's address is its only argument).
+struct TrajGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    void *block = nullptr;
+    uint64_t grid = 0;
+    int precision = 0;
+    uint64_t replays = 0;
+};
+
+void traj_graph_free(TrajGraph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+uint64_t traj_graph_replays(const TrajGraph *g) { return g ? g->replays : 0; }
+
+bool trajectories_supported(const Compiled &c) { return supported(c); }
+
+uint64_t traj_max_grid(uint64_t shots) { return std::max<uint64_t>(1, std::min<uint64_t>(shots, 2 * cu_count())); }
+
+uint64_t traj_workspace_bytes(const Compiled &c, int precision, uint64_t shots, uint64_t grid) {
+    const uint64_t tb = precision == 64 ? tables_bytes<double>(c, shots) : tables_bytes<float>(c, shots);
+    return grid * state_bytes_of(c, precision) + tb;
+}
+
+uint64_t traj_grid_for(const Compiled &c, int precision, uint64_t shots, uint64_t bytes) {
+    const uint64_t tb = precision == 64 ? tables_bytes<double>(c, shots) : tables_bytes<float>(c, shots);
+    if (bytes < tb) return 0;
+    return std::min<uint64_t>(traj_max_grid(shots), (bytes - tb) / state_bytes_of(c, precision));
+}
+
+}  // namespace zeno
+
+namespace {
+
+template <typename T>
+void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noise, uint64_t *out, double *secs,
+           TrajWorkspace &w) {
+    const int precision = sizeof(T) == 8 ? 64 : 32;
+    const uint64_t grid = traj_grid_for(c, precision, shots, w.bytes);
+    if (grid == 0)
+        throw TrajError{ZB_ERR_MEMORY, "a trajectory workspace of " + std::to_string(w.bytes) +
+                                           " B holds no state of " + std::to_string(c.n_qubits) + " qubits"};
+    const Tables<T> t = pack<T>(c);
+    char *base = static_cast<char *>(w.mem);
+    T *states = reinterpret_cast<T *>(base);
+    char *p = base + grid * state_bytes_of(c, precision);
+    auto carve = [&](size_t bytes) {
+        char *r = p;
+        p += up(bytes);
+        return r;
+    };
+    // the holder's fixed launch block when it has one, so its graph replays
+    char *own_block = carve(sizeof(dev::TrajLaunch<T>));
+    auto *dl = reinterpret_cast<dev::TrajLaunch<T> *>(w.launch_block ? w.launch_block : own_block);
+    auto *dops = reinterpret_cast<dev::TrajOp *>(carve(t.ops.size() * sizeof(dev::TrajOp)));
+    auto *dmats = reinterpret_cast<T *>(carve(t.mats.size() * sizeof(T)));
+    auto *dfm = reinterpret_cast<uint32_t *>(carve(t.fm.size() * 4));
+    auto *dout = reinterpret_cast<uint64_t *>(carve(std::max<uint64_t>(shots, 1) * 8));
+    if ((uint64_t)(p - base) > w.bytes) throw TrajError{ZB_ERR_MEMORY, "trajectory workspace too small"};
+    dev::TrajLaunch<T> L{};
+    L.states = states;
+    L.ops = dops;
+    L.fm = dfm;
+    L.mats = dmats;
+    L.out = dout;
+    L.seed = seed;
+    L.shots = shots;
+    L.n = c.n_qubits;
+    L.nops = (uint32_t)t.nops;
+    L.nfm = (uint32_t)t.nfm;
+    if (noise) {
+        L.nz = dev::TrajNoise{noise->depolarizing_1q, noise->depolarizing_2q, noise->bit_flip, noise->phase_flip,
+                              noise->amplitude_damping, noise->readout_flip_0to1, noise->readout_flip_1to0};
+        L.noisy = 1;
+    }
+    hipStream_t stream = static_cast<hipStream_t>(w.stream);
+    // Uploads on the launch stream, waited for before the host buffers go:
+    // a copy from pageable memory may return before its DMA lands.
+    hcheck(hipMemcpyAsync(dl, &L, sizeof(L), hipMemcpyHostToDevice, stream), "launch block");
+    hcheck(hipMemcpyAsync(dops, t.ops.data(), t.ops.size() * sizeof(dev::TrajOp), hipMemcpyHostToDevice, stream),
+           "ops");
+    hcheck(hipMemcpyAsync(dmats, t.mats.data(), t.mats.size() * sizeof(T), hipMemcpyHostToDevice, stream), "mats");
+    hcheck(hipMemcpyAsync(dfm, t.fm.data(), t.fm.size() * 4, hipMemcpyHostToDevice, stream), "fm");
+    hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    const auto t0 = std::chrono::steady_clock::now();
+    // One persistent launch, captured once per (launch block, grid,
+    // precision) and replayed while those repeat.
+    TrajGraph *&g = w.graph;
+    if (g && (g->block != (void *)dl || g->grid != grid || g->precision != precision)) {
+        traj_graph_free(g);
+        g = nullptr;
+    }
+    if (!g) {
+        auto *ng = new TrajGraph();
+        ng->block = dl;
+        ng->grid = grid;
+        ng->precision = precision;
+        try {
+            hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipLaunchKernelGGL(dev::k_trajectories<T>, dim3((unsigned)grid), dim3(dev::kBlock), 0, stream,
+                               (const dev::TrajLaunch<T> *)dl);
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(stream, &ng->graph);
+            hcheck(le, "k_trajectories");
+            hcheck(ce, "hipStreamEndCapture");
+            hcheck(hipGraphInstantiate(&ng->exec, ng->graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        } catch (...) {
+            traj_graph_free(ng);
+            throw;
+        }
+        g = ng;
+    } else {
+        ++g->replays;
+    }
+    hcheck(hipGraphLaunch(g->exec, stream), "hipGraphLaunch");
+    hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    if (secs) *secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    if (shots) {
+        hcheck(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, stream), "out copy");
+        hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    }
+}
+
+int checked_args(const Compiled &c, int precision) {
+    if (!supported(c)) {
         set_error("batched trajectories need <= 20 qubits and unitary/diagonal ops on <= 3 qubits");
         return ZB_ERR_UNSUPPORTED;
     }
@@ -164,12 +261,66 @@ extern "C" int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t
         set_error("no HIP device is available (the hip backend needs an AMD GPU)");
         return ZB_ERR_UNSUPPORTED;
     }
+    return ZB_OK;
+}
+
+}  // namespace
+
+namespace zeno {
+
+int run_trajectories_in(const Compiled &c, int precision, uint64_t shots, uint64_t seed, const zr_noise *noise,
+                        uint64_t *out, double *secs, TrajWorkspace &w) {
+    if (const int rc = checked_args(c, precision); rc != ZB_OK) return rc;
     try {
-        if (precision == 64) run_T<double>(c->c, shots, seed, noise, out, kernel_seconds);
-        else run_T<float>(c->c, shots, seed, noise, out, kernel_seconds);
+        if (precision == 64) run_T<double>(c, shots, seed, noise, out, secs, w);
+        else run_T<float>(c, shots, seed, noise, out, secs, w);
     } catch (const TrajError &e) {
         set_error(e.msg);
         return e.code;
     }
     return ZB_OK;
+}
+
+}  // namespace zeno
+
+extern "C" int zr_trajectories_supported(const zr_compiled *c) { return supported(c->c) ? 1 : 0; }
+
+extern "C" int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t shots, uint64_t seed,
+                                   const zr_noise *noise, uint64_t *out, double *kernel_seconds) {
+    if (const int rc = checked_args(c->c, precision); rc != ZB_OK) return rc;
+    // One workspace for the call, sized to the free device memory: up to two
+    // states per CU, fewer when they do not fit.
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) {
+        set_error("hipMemGetInfo failed");
+        return ZB_ERR_DEVICE;
+    }
+    const uint64_t margin = 256ull << 20;
+    const uint64_t avail = free_b > margin ? free_b - margin : 0;
+    const uint64_t grid = traj_grid_for(c->c, precision, shots, avail);
+    if (grid == 0) {
+        set_error("no trajectory workspace of " + std::to_string(c->c.n_qubits) + " qubits fits in " +
+                  std::to_string(free_b) + " B of free device memory");
+        return ZB_ERR_MEMORY;
+    }
+    TrajWorkspace w;
+    w.bytes = traj_workspace_bytes(c->c, precision, shots, grid);
+    hipStream_t stream = nullptr;
+    if (hipMalloc(&w.mem, w.bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        set_error("hipMalloc of a " + std::to_string(w.bytes) + " B trajectory workspace failed");
+        return ZB_ERR_MEMORY;
+    }
+    int rc = ZB_OK;
+    if (hipStreamCreateWithFlags(&stream, hipStreamNonBlocking) != hipSuccess) {
+        set_error("hipStreamCreate failed");
+        rc = ZB_ERR_DEVICE;
+    } else {
+        w.stream = stream;
+        rc = run_trajectories_in(c->c, precision, shots, seed, noise, out, kernel_seconds, w);
+        (void)hipStreamDestroy(stream);
+    }
+    traj_graph_free(w.graph);
+    (void)hipFree(w.mem);
+    return rc;
 }

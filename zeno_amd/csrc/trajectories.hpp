@@ -549,11 +549,34 @@ __device__ __forceinline__ int t_readout(const TrajNoise &nz, int bit, DevRng &r
     return bit;
 }
 
+// Everything one launch needs, in device memory, so that a captured graph
+// of the launch (its only argument is this block's address) replays
+// unchanged for any circuit whose launch block is written to the same place
+// and that runs at the same grid.
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_trajectories(T *__restrict__ ws, uint32_t n, const TrajOp *__restrict__ ops,
-                                                         uint32_t nops, const uint32_t *__restrict__ fm, uint32_t nfm,
-                                                         const T *__restrict__ mats, TrajNoise nz, int noisy,
-                                                         uint64_t seed, uint64_t shots, uint64_t *__restrict__ out) {
+struct TrajLaunch {
+    T *states;           // gridDim.x states of 2 * 2^n values, packed
+    const TrajOp *ops;
+    const uint32_t *fm;  // final measures: (qubit, clbit) pairs
+    const T *mats;
+    uint64_t *out;       // per-shot clbit words
+    uint64_t seed, shots;
+    TrajNoise nz;
+    uint32_t n, nops, nfm;
+    int32_t noisy;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__restrict__ L) {
+    T *__restrict__ ws = L->states;
+    const uint32_t n = L->n, nops = L->nops, nfm = L->nfm;
+    const TrajOp *__restrict__ ops = L->ops;
+    const uint32_t *__restrict__ fm = L->fm;
+    const T *__restrict__ mats = L->mats;
+    uint64_t *__restrict__ out = L->out;
+    const TrajNoise nz = L->nz;
+    const int noisy = L->noisy;
+    const uint64_t seed = L->seed, shots = L->shots;
     __shared__ double red[9 * kBlock];
     __shared__ T smat[2 << (2 * kTrajMaxK)];
     __shared__ T spauli[16];

@@ -13,7 +13,7 @@ from typing import Optional
 
 import numpy as np
 
-from . import ir
+from . import _lib, ir
 from ._lib import last_error, lib, ptr
 from .backend import HipBackend, MemoryBudget, ZenoError, _check
 
@@ -161,45 +161,50 @@ class RunResult:
     notices: list
 
 
+class _MemPlan(C.Structure):
+    """zr_mem_plan: mem::MemPlan plus the fields of Error::Memory."""
+
+    _fields_ = [("n_qubits", C.c_uint32), ("precision", C.c_int32), ("needed_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("max_qubits", C.c_uint32)]
+
+
+def _memory_error(p: "_MemPlan") -> MemoryBudget:
+    e = MemoryBudget(_lib.last_error())
+    e.n_qubits, e.precision, e.needed_bytes = int(p.n_qubits), int(p.precision), int(p.needed_bytes)
+    e.budget_bytes, e.max_qubits = int(p.budget_bytes), int(p.max_qubits)
+    return e
+
+
 def state_bytes(n_qubits: int, precision: int) -> int:
     """mem::state_bytes (src/mem.rs:79-81): 8 B/amp in f32, 16 B in f64."""
-    return (1 << n_qubits) * (16 if precision == 64 else 8)
+    p = _MemPlan()
+    lib().zr_plan(n_qubits, precision, 0, C.byref(p))
+    return int(p.needed_bytes)
 
 
 def max_qubits(budget: int, precision: int) -> int:
     """mem::max_qubits (src/mem.rs:84-93)."""
-    n = 0
-    while state_bytes(n + 1, precision) <= budget and n < 60:
-        n += 1
-    return n
+    p = _MemPlan()
+    lib().zr_plan(0, precision, budget, C.byref(p))
+    return int(p.max_qubits)
 
 
-def device_budget_bytes(fraction: float = 0.9) -> int:
-    """Budget from HBM instead of RAM (`ZENO_MEM_BYTES` overrides, as in
-    src/mem.rs:11-17)."""
-    env = os.environ.get("ZENO_MEM_BYTES")
-    if env:
-        try:
-            return int(env)
-        except ValueError:
-            pass
-    from .backend import device_count, device_memory
-
-    if device_count() > 0:
-        free, _total = device_memory()
-        return int(free * fraction)
-    return 288 << 30
+def budget_bytes(mem_limit: Optional[int] = None, fraction: float = 0.75) -> int:
+    """mem::budget_bytes (src/mem.rs:103-109) on HBM: an explicit limit wins,
+    otherwise `fraction` of the device's total HBM (`ZENO_MEM_BYTES`
+    replaces that total, as it replaces RAM in src/mem.rs:11-17)."""
+    return int(lib().zr_budget_bytes(mem_limit or 0, fraction))
 
 
 def plan(n_qubits: int, precision: int, budget: int):
-    """mem::plan (src/mem.rs:114-130)."""
-    need = state_bytes(n_qubits, precision)
-    if need > budget:
-        raise MemoryBudget(
-            f"state vector for {n_qubits} qubits at f{precision} needs {need} B but the budget is {budget} B "
-            f"(max {max_qubits(budget, precision)} qubits at this precision)"
-        )
-    return need
+    """mem::plan (src/mem.rs:114-130): the needed bytes, or MemoryBudget
+    carrying Error::Memory's fields."""
+    p = _MemPlan()
+    rc = lib().zr_plan(n_qubits, precision, budget, C.byref(p))
+    if rc == _lib.ZB_ERR_MEMORY:
+        raise _memory_error(p)
+    _check(rc, "plan")
+    return int(p.needed_bytes)
 
 
 @dataclasses.dataclass
@@ -211,25 +216,23 @@ class RunOptions:
     precision: Optional[int] = None  # None = auto: f64 if it fits, else f32
     fusion_max: Optional[object] = None  # None = 1 (the CPU path's default); "auto" = autotuned width
     mem_limit: Optional[int] = None
+    mem_fraction: float = 0.75  # src/exec.rs:65
     want_statevector: bool = False
     noise: Optional[NoiseModel] = None
     mode: int = HipBackend.TILED
 
 
 def resolve_precision(n_qubits: int, opts: RunOptions):
-    """exec.rs:261-304 with the HBM budget."""
-    budget = opts.mem_limit if opts.mem_limit is not None else device_budget_bytes()
-    notices = []
-    if opts.precision is not None:
-        plan(n_qubits, opts.precision, budget)
-        return opts.precision, notices
-    try:
-        plan(n_qubits, 64, budget)
-        return 64, notices
-    except MemoryBudget:
-        plan(n_qubits, 32, budget)
-        notices.append(f"auto-precision: {n_qubits} qubits exceed the f64 budget, using f32 (8 B/amplitude)")
-        return 32, notices
+    """resolve_precision (src/exec.rs:261-304) on the HBM budget, through
+    zr_resolve_precision."""
+    budget = budget_bytes(opts.mem_limit, opts.mem_fraction)
+    prec, p, note = C.c_int(0), _MemPlan(), C.create_string_buffer(256)
+    rc = lib().zr_resolve_precision(n_qubits, opts.precision or 0, budget, C.byref(prec), C.byref(p), note, 256)
+    if rc == _lib.ZB_ERR_MEMORY:
+        raise _memory_error(p)
+    _check(rc, "resolve_precision")
+    notices = [note.value.decode()] if note.value else []
+    return int(prec.value), notices
 
 
 def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
@@ -283,7 +286,10 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
     if noise is not None:
         notices.append("noise: trajectory sampling — per-shot execution, fusion disabled")
     if not c.final_measures and not c.dynamic and opts.shots > 0:
-        notices.append("circuit has no measurements: counts are empty (use --statevector to inspect the state)")
+        # run_per_shot's text on the noisy path (src/exec.rs:476-482), where a
+        # statevector cannot be asked for; run_sampled's otherwise (:426-433)
+        notices.append("circuit has no measurements: counts are empty" if noise is not None else
+                       "circuit has no measurements: counts are empty (use --statevector to inspect the state)")
     counts = Counts()
     for k, v in raw.items():
         key = format_key(k, c.cregs)
@@ -315,6 +321,35 @@ def run_trajectories(c: Compiled, shots: int, seed: int, precision: int = 64, no
     )
     _check(rc, "run_trajectories")
     return out[:shots], secs.value
+
+
+def batch_supported(c: Compiled, precision: int, shots: int) -> bool:
+    """Whether `run_batch` (and the server's batcher) takes the circuit:
+    static, <= 13 qubits in f64 / 14 in f32, dense ops on <= 3 / 4 qubits,
+    <= 2^17 shots."""
+    return bool(lib().zr_batch_supported(c.handle, precision, shots))
+
+
+def run_batch(runs):
+    """Many small static circuits in one persistent launch: `runs` is a list
+    of (Compiled, precision, shots, seed); returns ([raw counts per run],
+    seconds).  Each run's counts equal run_compiled's for the same inputs."""
+    runs = list(runs)
+    m = len(runs)
+    hs = (C.c_void_p * max(m, 1))(*[r[0].handle for r in runs])
+    prec = np.asarray([r[1] for r in runs] or [64], np.int32)
+    shots = np.asarray([r[2] for r in runs] or [0], np.uint64)
+    seeds = np.asarray([r[3] for r in runs] or [0], np.uint64)
+    words = np.zeros(max(int(shots[:m].sum()) if m else 0, 1), np.uint64)
+    secs = C.c_double(0.0)
+    _check(lib().zr_run_batch(hs, ptr(prec), ptr(shots), ptr(seeds), m, ptr(words), C.byref(secs)), "run_batch")
+    out, at = [], 0
+    for c, _p, s, _sd in runs:
+        w = words[at:at + s] if c.final_measures else words[:0]
+        at += s
+        v, k = np.unique(w, return_counts=True)
+        out.append({int(a): int(b) for a, b in zip(v, k)})
+    return out, secs.value
 
 
 def _splitmix64(x: int) -> int:

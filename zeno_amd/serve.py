@@ -44,11 +44,18 @@ class Served:
 
 
 class Server:
-    def __init__(self, pool_bytes: int = 0, slots: int = 4, mode: int = HipBackend.TILED):
+    """`batch`: small static requests (see `batch_supported`) run together,
+    up to this many per persistent launch, beside the slots; 0 sends every
+    request to a slot (with slots=1: one at a time)."""
+
+    def __init__(self, pool_bytes: int = 0, slots: int = 4, mode: int = HipBackend.TILED,
+                 batch: Optional[int] = None):
         h = C.c_void_p()
         _check(lib().zs_create(pool_bytes, slots, mode, C.byref(h)), "serve")
         self._h = h
         self._cregs = {}
+        if batch is not None:
+            _check(lib().zs_set_batch(self._h, batch), "serve batch")
 
     def close(self):
         if getattr(self, "_h", None):
@@ -97,3 +104,8 @@ class Server:
         s = np.zeros(4, np.int64)
         lib().zs_status(self._h, ptr(s))
         return {"queued": int(s[0]), "running": int(s[1]), "largest_free": int(s[2]), "pool_bytes": int(s[3])}
+
+    def counters(self) -> dict:
+        s = np.zeros(4, np.int64)
+        lib().zs_counters(self._h, ptr(s))
+        return {"batches": int(s[0]), "batched": int(s[1]), "graph_replays": int(s[2]), "max_batch": int(s[3])}
