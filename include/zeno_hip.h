@@ -111,6 +111,21 @@ int zb_reset_all(zb_backend *h);
  * shots, when they are sorted on the device). */
 int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out);
 
+/* zb_sample without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * that must not call hipMalloc or hipFree, such as tight serving loops or
+ * harnesses that forbid it).  The same kernels and the same result as
+ * zb_sample (indices equal for the same state, shots and seed), with the
+ * sampler's device scratch — chunk sums, prefix, draws, routing, sort temp —
+ * in a fixed 64 MiB region per GPU that the library holds until the process
+ * ends.  That is enough for 2^20 shots at any n <= 35; a request needing
+ * more fails with ZB_ERR_ARG (the message names the bytes needed) before any
+ * GPU work.  Every call on one GPU shares that region: calls that overlap on
+ * one GPU (from several threads or handles) race and must be serialised by
+ * the caller. */
+int zb_sample_noalloc(zb_backend *h, uint64_t shots, uint64_t seed,
+                      uint64_t *out);
+
 /* Backend::statevector (src/exec.rs:155; src/state.rs:134-140): writes
  * 2^n interleaved complex128 values to `out` (host memory). */
 int zb_statevector(zb_backend *h, double *out);

@@ -31,6 +31,7 @@ SIGNATURES = {
     "zb_measure": (I32, [P, U32, D]),
     "zb_reset_all": (I32, [P]),
     "zb_sample": (I32, [P, U64, U64, P]),
+    "zb_sample_noalloc": (I32, [P, U64, U64, P]),
     "zb_statevector": (I32, [P, P]),
     "zb_norm_sqr": (D, [P]),
     "zb_finish": (I32, [P]),

@@ -35,6 +35,18 @@ namespace {
 
 constexpr size_t kArenaBytes = 64ull << 20;
 
+// Scratch of the allocation-free sampler (zb_sample_noalloc): one copy of
+// this array per GPU, held until the process ends.  Host code never uses its
+// name as an address: each device's copy is looked up once, with that device
+// current, when the first handle on it is set up, and kept in g_noalloc_ptr.
+// Every output slot the sampler reads back is written by the same call, so
+// nothing in it has to start at zero.
+constexpr size_t kNoallocScratchBytes = 64ull << 20;
+static __device__ __align__(256) char scratch[kNoallocScratchBytes];
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
 struct HipError {
     std::string msg;
 };
@@ -1212,11 +1224,38 @@ void set_shape(zb_backend *h, uint32_t n_qubits, int precision) {
     h->name = precision == 64 ? "hip-f64" : "hip-f32";
 }
 
+// The device's copy of the allocation-free sampler's scratch array, looked
+// up once per device with that device current.  A failed lookup is kept as
+// an error message for zb_sample_noalloc (every other entry point works
+// without the array).
+std::string g_noalloc_err[kMaxDevices];
+
+void noalloc_setup(int device) {
+    if (device < 0 || device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        (void)hipGetLastError();
+        g_noalloc_err[device] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+        return;
+    }
+    g_noalloc_ptr[device] = p;
+}
+
+void *noalloc_ptr(int device) {
+    if (device < 0 || device >= kMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
 // Stream, parameter arena, pinned result slot and layout table: everything
 // but the state (the layout table too, so that no read of the state has to
 // allocate later).
 void create_resources(zb_backend *h) {
     check(hipGetDevice(&h->device), "hipGetDevice");
+    noalloc_setup(h->device);
     check(hipMalloc((void **)&h->lut_dev, 5 * 256 * sizeof(uint64_t)), "hipMalloc layout");
     check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
     check(hipMalloc((void **)&h->arena_dev, kArenaBytes), "hipMalloc arena");
@@ -1429,6 +1468,27 @@ int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
     return guarded([&] {
         if (h->prec == 64) sample_T<double>(h, shots, seed, out);
         else sample_T<float>(h, shots, seed, out);
+    });
+}
+
+int zb_sample_noalloc(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out) {
+    const size_t need = sample_scratch_bytes(h->len, shots);
+    if (need > kNoallocScratchBytes) {
+        set_error("zb_sample_noalloc: " + std::to_string(shots) + " shots from " + std::to_string(h->n) +
+                  " qubits need " + std::to_string(need) + " B of scratch, more than its " +
+                  std::to_string(kNoallocScratchBytes) + " B (use zb_sample)");
+        return ZB_ERR_ARG;
+    }
+    char *p = static_cast<char *>(noalloc_ptr(h->device));
+    if (!p) {
+        std::lock_guard<std::mutex> lk(g_noalloc_mu);
+        set_error("zb_sample_noalloc: no scratch address for device " + std::to_string(h->device) + ": " +
+                  (h->device >= 0 && h->device < kMaxDevices ? g_noalloc_err[h->device] : std::string("out of range")));
+        return ZB_ERR_DEVICE;
+    }
+    return guarded([&] {
+        if (h->prec == 64) sample_T<double>(h, shots, seed, out, p);
+        else sample_T<float>(h, shots, seed, out, p);
     });
 }
 
