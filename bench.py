@@ -74,41 +74,174 @@ def run_gpu(n, comp, prec, mode, steps, warmup, knobs=None):
     return be, sum(step_times) / len(step_times), per_class, circuit_bytes
 
 
+def host_threads():
+    """Host threads for the CPU legs: every core this process may run on
+    (its affinity set), limited to OMP_NUM_THREADS when that is set, as it is
+    on a GPU box to the host cores that come with one GPU."""
+    n = len(os.sched_getaffinity(0))
+    try:
+        omp = int(os.environ.get("OMP_NUM_THREADS", "0"))
+    except ValueError:
+        omp = 0
+    return max(1, min(n, omp) if omp > 0 else n)
+
+
 def cpu_baseline(args, prog):
+    """The CPU path (the oracle's restatement of the reference's scalar
+    kernels) on every host thread, timed on a bounded sample of the same
+    circuit: the ops up to and including the first RZZ ladder (five U3 +
+    CX-matching layers, then the ladder's diagonals).  The circuit estimate
+    adds each op kind's measured mean time for every op of that kind."""
     import oracle
 
     n = args.n
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    cores = host_threads()
     oracle.set_threads(cores)
     comp = oracle.Compiled(prog, n, fusion=1)  # the CPU path's own default
-    m = min(args.cpu_ops, len(comp))
+    ops = comp.ops()
+    first_diag = next((i for i, o in enumerate(ops) if o["kind"] == 1), len(ops) - 1)
+    m = first_diag
+    while m < len(ops) and ops[m]["kind"] == 1:
+        m += 1
+    m = max(1, min(m, len(ops), args.cpu_ops or len(ops)))
     t0 = time.perf_counter()
     st = oracle.State(n, 64)
     t_zero = time.perf_counter() - t0
-    t1 = time.perf_counter()
-    st.apply_compiled(comp, 0, m)
-    t_ops = time.perf_counter() - t1
-    ops = comp.ops()[:m]
+    per_kind = {}
+    for i in range(m):
+        t1 = time.perf_counter()
+        st.apply_compiled(comp, i, i + 1)
+        dt = time.perf_counter() - t1
+        k = per_kind.setdefault(ops[i]["kind"], [0, 0.0])
+        k[0] += 1
+        k[1] += dt
+    t_ops = sum(v[1] for v in per_kind.values())
     unit = (1 << n) * 8
-    nbytes = sum((2 if o["kind"] == 2 else 4) * unit for o in ops)
-    all_ops = comp.ops()
-    per_op = t_ops / m
-    est = t_zero + per_op * len(all_ops)
+    nbytes = sum((2 if o["kind"] == 2 else 4) * unit for o in ops[:m])
+    mean = {k: v[1] / v[0] for k, v in per_kind.items()}
+    fallback = t_ops / m
+    counts = {}
+    for o in ops:
+        counts[o["kind"]] = counts.get(o["kind"], 0) + 1
+    est = t_zero + sum(c * mean.get(k, fallback) for k, c in counts.items())
+    names = {0: "unitary", 1: "diagonal", 2: "cx"}
     del st
     return {
         "value": round(nbytes / t_ops / 1e9, 3),
         "unit": "GB/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"first {m} of {len(all_ops)} ops of the same {n}q circuit at the CPU path's default fusion 1, "
-        f"f64, oracle/zeno_oracle.c on {cores} threads; est. {est:.1f} s/circuit "
-        f"(zero {t_zero:.2f} s + {per_op:.3f} s/op)",
+        "sample": f"first {m} of {len(ops)} ops of the same {n}q circuit (five U3 + CX-matching layers and the "
+        f"first RZZ ladder) at the CPU path's default fusion 1, f64, oracle/zeno_oracle.c on {cores} threads; "
+        f"est. {est:.1f} s/circuit from per-kind means "
+        + ", ".join(f"{names.get(k, k)} {1e3 * t:.1f} ms" for k, t in sorted(mean.items())),
         "s_per_circuit_est": round(est, 3),
+        "sample_s": round(t_zero + t_ops, 3),
     }
+
+
+def _median(xs):
+    xs = sorted(xs)
+    return xs[len(xs) // 2]
+
+
+def bench_small_configs(args):
+    """The reference's published workloads, end to end in its sim_time scope
+    (state allocation, every op, finish, sampling; src/exec.rs:372-381) on
+    the GPU and on the CPU path: the 20-qubit depth-12 random circuit at
+    fusion 2 with 1024 shots (both generators), the seeded U3 layer + QFT on
+    24 qubits and the 26-qubit depth-12 random circuit at fusion 2, 3 and 4
+    with 1M shots.  GPU: median of 3 after a warm-up run; CPU: median of 3
+    where one run is short, else one run.  Counts must be identical."""
+    import oracle
+
+    import zeno_amd as z
+
+    cores = host_threads()
+    oracle.set_threads(cores)
+    cases = [("20q d12 brickwork, K=2, 1024 shots", oracle.gen_random(20, 12, 42), 20, 2, 1024, 42, 3),
+             ("20q d12 random matching, K=2, 1024 shots", oracle.gen_matching(20, 12, 42), 20, 2, 1024, 42, 3),
+             ("24q seeded U3 layer + QFT, K=2, 1M shots", oracle.gen_u3_qft(24, 11), 24, 2, 1_000_000, 42, 3),
+             ("26q d12 brickwork, K=2, 1M shots", oracle.gen_random(26, 12, 26), 26, 2, 1_000_000, 7, 1),
+             ("26q d12 brickwork, K=3, 1M shots", oracle.gen_random(26, 12, 26), 26, 3, 1_000_000, 7, 1),
+             ("26q d12 brickwork, K=4, 1M shots", oracle.gen_random(26, 12, 26), 26, 4, 1_000_000, 7, 1)]
+    rows = []
+    for label, body, n, k, shots, seed, cpu_reps in cases:
+        prog = oracle.program([body, oracle.measure_all(n)])
+        gc = z.compile(prog, n, fusion=k)
+        opts = z.RunOptions(shots=shots, seed=seed, precision=64, fusion_max=k)
+        z.run_compiled(gc, opts)  # warm-up
+        gt = []
+        for _ in range(3):
+            r = z.run_compiled(gc, opts)
+            gt.append(r.sim_time)
+        row = {"workload": label, "ops": len(gc), "gpu_s": round(_median(gt), 5),
+               "gpu_runs_s": [round(x, 5) for x in gt]}
+        if not args.no_cpu:
+            oc = oracle.Compiled(prog, n, fusion=k)
+            ct, want = [], None
+            for _ in range(cpu_reps):
+                want, secs = oracle.run(oc, shots, seed, 64)
+                ct.append(secs)
+            row["cpu_s"] = round(_median(ct), 4)
+            row["cpu_runs_s"] = [round(x, 4) for x in ct]
+            row["cpu_threads"] = cores
+            row["gpu_speedup"] = round(row["cpu_s"] / row["gpu_s"], 1)
+            row["counts_identical"] = r.raw_counts == want
+        _log(f"small config {label}: {row}")
+        rows.append(row)
+    return rows
 
 
 NOISE_C4 = {"depolarizing_1q": 1e-3, "depolarizing_2q": 1e-2, "amplitude_damping": 5e-3,
             "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+
+
+# The reference's own head-to-head table (docs/PERFORMANCE.md:14-22): Apple
+# M4 Pro, sim_time, 1024 shots, default fusion; seconds for cpu f64, cpu f32
+# and metal f32 (None where it publishes none).
+PUBLISHED = [
+    ("random 20q, depth 12", "random", 20, 12, (0.040, 0.032, None)),
+    ("random 24q, depth 12", "random", 24, 12, (0.900, 0.450, 0.253)),
+    ("random 26q, depth 12", "random", 26, 12, (3.86, 2.05, 0.95)),
+    ("random 24q, depth 40", "random", 24, 40, (2.99, 1.58, 0.73)),
+    ("QFT 24q (no final swaps)", "qft", 24, 0, (0.258, 0.210, 0.140)),
+]
+
+
+def bench_published(args):
+    """The reference's published workloads as it runs them (brickwork random
+    circuits and the QFT from |0...0>, 1024 shots, fusion 1, sim_time scope)
+    on the GPU in f64 and f32, median of 3 after a warm-up, beside its M4
+    Pro numbers and, in f64, this repository's CPU path on the same host."""
+    import oracle
+
+    import zeno_amd as z
+
+    cores = host_threads()
+    oracle.set_threads(cores)
+    rows = []
+    for label, gen, n, depth, (cpu64, cpu32, metal32) in PUBLISHED:
+        body = oracle.gen_random(n, depth, 42) if gen == "random" else oracle.gen_qft(n)
+        prog = oracle.program([body, oracle.measure_all(n)])
+        gc = z.compile(prog, n, fusion=1)
+        row = {"workload": label, "ops": len(gc), "published_s": {"m4pro_cpu_f64": cpu64, "m4pro_cpu_f32": cpu32,
+                                                                   "m4pro_metal_f32": metal32}}
+        for prec in (64, 32):
+            opts = z.RunOptions(shots=1024, seed=42, precision=prec, fusion_max=1)
+            z.run_compiled(gc, opts)
+            ts = [z.run_compiled(gc, opts).sim_time for _ in range(3)]
+            row[f"gpu_f{prec}_s"] = round(_median(ts), 5)
+        row["speedup_f64_vs_m4pro_cpu_f64"] = round(cpu64 / row["gpu_f64_s"], 1)
+        if metal32:
+            row["speedup_f32_vs_m4pro_metal_f32"] = round(metal32 / row["gpu_f32_s"], 1)
+        if not args.no_cpu:
+            _, secs = oracle.run(oracle.Compiled(prog, n, fusion=1), 1024, 42, 64)
+            row["cpu_f64_s_here"] = round(secs, 4)
+            row["cpu_threads"] = cores
+        _log(f"published workload {label}: {row}")
+        rows.append(row)
+    return rows
 
 
 def bench_trajectories(args):
@@ -131,21 +264,27 @@ def bench_trajectories(args):
         import oracle
 
         m = args.traj_cpu_shots
+        cores = host_threads()
         oc = oracle.Compiled(prog, n, fusion=0)
         t0 = time.perf_counter()
-        oracle.run(oc, m, args.seed, prec=32, noise=NOISE_C4)
+        words = oracle.run_shots(oc, m, args.seed, 32, NOISE_C4, cores)
         dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(m / dt, 2), "unit": "trajectories/s", "cores": 1, "kind": "port",
+        gw, _ = z.run_trajectories(c, m, args.seed, 32, nz)
+        out["cpu_baseline"] = {"value": round(m / dt, 2), "unit": "trajectories/s", "cores": cores, "kind": "port",
                                "sample": f"first {m} of the {shots} trajectories, oracle/zeno_oracle.c per-shot "
-                                         f"sampler, f32, 1 thread"}
+                                         f"path folded over {cores} threads with a state each "
+                                         f"(src/exec.rs:507-528), f32",
+                               "words_identical": bool((gw == words).all())}
     return out
 
 
 def bench_serve(args):
     """Mixed-size serving: a Poisson stream of seeded random circuits (n ~
     U{10..28}, depth ~ U{8..40}, 1000 shots, f32 or f64 each) through the
-    continuous-batching server, against the same stream served one at a
-    time (one slot)."""
+    continuous-batching server (slots for the large requests, one persistent
+    launch per batch for the small static ones), against the same stream
+    served one at a time (one slot, no batching).  Latency percentiles skip
+    the first `--serve-warmup` requests of each run."""
     import threading
 
     import numpy as np
@@ -160,13 +299,15 @@ def bench_serve(args):
         prec = 32 if rng.integers(0, 2) else 64
         prog = z.Circuit(n).extend(z.random_circuit(n, depth, 1000 + i).to_program()[0]).measure_all().to_program()[0]
         reqs.append((z.compile(prog, n, fusion=1), prec, 1000, 5000 + i))
+    small = sum(1 for c, p, s, _ in reqs if z.batch_supported(c, p, s))
+    skip = min(args.serve_warmup, max(0, len(reqs) - 1))
     results = {}
     for rate in [float(x) for x in args.serve_rates.split(",") if x]:
         gaps = rng.exponential(1.0 / rate, len(reqs))
         arrivals = np.cumsum(gaps) - gaps[0]
         row = {}
-        for label, slots in (("served", args.serve_slots), ("one_at_a_time", 1)):
-            with z.Server(args.serve_pool << 30, slots) as srv:
+        for label, slots, batch in (("served", args.serve_slots, None), ("one_at_a_time", 1, 0)):
+            with z.Server(args.serve_pool << 30, slots, batch=batch) as srv:
                 ids = [None] * len(reqs)
 
                 def feed():
@@ -181,17 +322,20 @@ def bench_serve(args):
                 th.start()
                 th.join()
                 done = [srv.wait(i) for i in ids]
-            lat = np.array([d.latency for d in done]) * 1e3
+                cnt = srv.counters()
+            lat = np.array([d.latency for d in done[skip:]]) * 1e3
             span = max(d.t_end for d in done) - min(d.t_submit for d in done)
             row[label] = {"slots": slots, "circuits_per_s": round(len(done) / span, 2),
                           "p50_ms": round(float(np.percentile(lat, 50)), 2),
                           "p99_ms": round(float(np.percentile(lat, 99)), 2),
-                          "mean_queue_ms": round(float(np.mean([d.queued for d in done])) * 1e3, 2)}
+                          "mean_queue_ms": round(float(np.mean([d.queued for d in done[skip:]])) * 1e3, 2),
+                          "batches": cnt["batches"], "batched": cnt["batched"]}
             _log(f"serve rate={rate} {label}: {row[label]}")
         results[str(rate)] = row
-    return {"workload": f"{len(reqs)} requests, Poisson arrivals, n ~ U{{10..28}}, depth ~ U{{8..40}}, "
+    return {"workload": f"{len(reqs)} requests per rate, Poisson arrivals, n ~ U{{10..28}}, depth ~ U{{8..40}}, "
                         f"1000 shots, f32/f64 at random, fusion 1 (compiled before arrival), pool "
-                        f"{args.serve_pool} GiB",
+                        f"{args.serve_pool} GiB; {small} of them small enough to batch; percentiles over requests "
+                        f"{skip + 1}..{len(reqs)}",
             "rates": results}
 
 
@@ -205,7 +349,8 @@ def main():
     ap.add_argument("--fusion", type=int, default=1)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--mode", type=int, default=1, help="0 exact per-op sweeps, 1 tiled")
-    ap.add_argument("--cpu-ops", type=int, default=8)
+    ap.add_argument("--cpu-ops", type=int, default=0,
+                    help="cap on the CPU baseline's sampled ops (0: through the first RZZ ladder)")
     ap.add_argument("--also", default="", help="extra fusion:mode pairs to time once, e.g. 5:0,5:1")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -213,12 +358,16 @@ def main():
                     "that scales rocprofv3 FETCH_SIZE/WRITE_SIZE")
     ap.add_argument("--tune", default="", help="tiled-kernel knobs, e.g. prefetch=1,win_grid=1024")
     ap.add_argument("--traj-shots", type=int, default=4096)
-    ap.add_argument("--traj-cpu-shots", type=int, default=24)
-    ap.add_argument("--serve-requests", type=int, default=80)
-    ap.add_argument("--serve-rates", default="50", help="Poisson arrival rates (req/s), comma separated")
+    ap.add_argument("--traj-cpu-shots", type=int, default=256)
+    ap.add_argument("--serve-requests", type=int, default=200)
+    ap.add_argument("--serve-warmup", type=int, default=20, help="requests left out of the latency percentiles")
+    ap.add_argument("--serve-rates", default="5,10,20,50,100,200",
+                    help="Poisson arrival rates (req/s), comma separated")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=48, help="serving pool in GiB")
     ap.add_argument("--no-extra", action="store_true", help="skip the trajectory and serving legs")
+    ap.add_argument("--no-small", action="store_true",
+                    help="skip the 20-, 24- and 26-qubit end-to-end legs (GPU and CPU)")
     ap.add_argument("--no-variants", action="store_true",
                     help="skip the opt-in fma / mfma variants (within tolerance, not bit-identical)")
     args = ap.parse_args()
@@ -251,6 +400,15 @@ def main():
         ms = z.calibration_copy(2 << n, reps=2)
         nb = (2 << n) * 8
         _log(f"calibration copy (k_copy): {2 * nb} bytes moved per copy ({nb} read, same written), {ms:.3f} ms")
+
+    # What a plain read + write pass of the state's size reaches here, for
+    # the roofline beside the data sheet's 8 TB/s.
+    probes = {}
+    for kind, label in ((0, "8B"), (1, "16B"), (2, "16B_x4"), (3, "16B_x4_nt")):
+        ms = z.copy_probe(2 << n, kind, reps=3)
+        probes[label] = round(2 * (2 << n) * 8 / (ms * 1e-3) / 1e9, 1)
+    _log(f"streaming copy probes (GB/s, read + write of {2 * (2 << n) * 8} B): {probes}")
+    ceiling = max(probes.values())
 
     be64, s64, cls64, bytes64 = run_gpu(n, comp, 64, args.mode, args.steps, args.warmup)
     be32, s32, cls32, bytes32 = run_gpu(n, comp, 32, args.mode, args.steps, args.warmup)
@@ -362,17 +520,25 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": traffic,
             "avg_us": round(avg_ms * 1e3, 2),
+            "stream_ceiling": ceiling,
+            "frac_of_stream_ceiling": round(achieved / ceiling, 4),
+            "copy_probes_gbs": probes,
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_small:
+        _log("end-to-end 20/24/26-qubit workloads ...")
+        out["small_configs"] = bench_small_configs(args)
+        _log("the reference's published workloads ...")
+        out["published_workloads"] = bench_published(args)
     if not args.no_extra:
         _log("batched trajectories ...")
         out["trajectories"] = bench_trajectories(args)
         if args.serve_requests > 0:
             _log("serving ...")
             out["serve"] = bench_serve(args)
-            first = next(iter(out["serve"]["rates"].values()))
-            out["p99_latency_ms_mixed_serve"] = first["served"]["p99_ms"]
+            rates = out["serve"]["rates"]
+            out["p99_latency_ms_mixed_serve"] = {r: v["served"]["p99_ms"] for r, v in rates.items()}
     print(json.dumps(out), flush=True)
 
 

@@ -179,6 +179,13 @@ int zb_mem_info(uint64_t *free_bytes, uint64_t *total_bytes);
  * (8 B read + 8 B written per element) that scales hardware byte
  * counters.  Allocates and frees its own buffers. */
 int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms);
+/* Streaming-copy probes of n_doubles f64 values (same bytes as the
+ * calibration copy), average ms over `reps` after one untimed copy:
+ * kind 0 = zb_calibration_copy (8-byte accesses); 1 = 16-byte accesses;
+ * 2 = 16-byte accesses, four loads in flight per lane before the stores;
+ * 3 = as 2 with non-temporal stores.  What a read + write pass reaches on
+ * the device, beside the 8 TB/s of the data sheet. */
+int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms);
 /* The backend's qubit layout: phys[q] = the bit of the state index that
  * holds circuit qubit q (n entries).  Every zb_* call takes and returns
  * circuit (logical) order; only zb_state_device_ptrs exposes the physical

@@ -13,6 +13,7 @@ from .backend import (
     Unsupported,
     ZenoError,
     calibration_copy,
+    copy_probe,
     device_count,
     device_memory,
 )
@@ -44,7 +45,7 @@ from .serve import Served, Server
 from . import qasm, tune
 
 __all__ = [
-    "Circuit", "Compiled", "calibration_copy", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
+    "Circuit", "Compiled", "calibration_copy", "copy_probe", "device_count", "device_memory", "Counts", "DeviceError", "HipBackend", "InvalidCircuit", "MemoryBudget",
     "NoiseModel", "RunOptions", "RunResult", "Simulator", "Unsupported", "ZenoError", "compile",
     "format_key", "ir", "qasm", "tune", "matching_circuit", "max_qubits", "plan", "budget_bytes", "resolve_precision", "batch_supported", "run_batch", "qft", "random_circuit",
     "Served", "Server", "run_compiled", "run_program", "run_trajectories", "state_bytes", "trajectories_supported", "u3_qft",

@@ -63,6 +63,7 @@ SIGNATURES = {
     "zb_device_count": (I32, []),
     "zb_mem_info": (I32, [P, P]),
     "zb_calibration_copy": (I32, [U64, I32, P]),
+    "zb_copy_probe": (I32, [U64, I32, I32, P]),
     "zb_layout": (I32, [P, P]),
     "zb_create_bound": (I32, [U32, I32, P, P, P]),
     "zb_rebind": (I32, [P, U32, I32, P, P]),

@@ -68,6 +68,13 @@ def calibration_copy(n_doubles: int, reps: int = 3) -> float:
     return float(ms.value)
 
 
+def copy_probe(n_doubles: int, kind: int, reps: int = 3) -> float:
+    """Average ms of a streaming copy probe (zb_copy_probe kinds 0..3)."""
+    ms = C.c_double()
+    _check(lib().zb_copy_probe(n_doubles, reps, kind, C.byref(ms)), "copy_probe")
+    return float(ms.value)
+
+
 class HipBackend:
     """make_backend(n, precision, BackendChoice::Hip) (src/exec.rs:306-331)."""
 

@@ -451,8 +451,9 @@ void exact_cx_T(zb_backend *h, uint32_t c, uint32_t t) {
 // so do dense ops on 5 or 6 qubits and every op of a state under 12 qubits.
 uint32_t tile_width(const zb_backend *h) { return h->tile_bits ? h->tile_bits : 12; }
 uint32_t tile_bits(const zb_backend *h) { return std::min<uint32_t>(h->n, tile_width(h)); }
-// Tiled windows need n >= 12 (full tiles) and 32-bit lane offsets (n <= 33).
-bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= tile_width(h) && h->n <= 33; }
+// Tiled windows need n >= 12 (full tiles); lane offsets and rows are 64-bit
+// state indices, so every size the backend holds (to 35 qubits) is tiled.
+bool tiled(const zb_backend *h) { return h->mode == 1 && h->n >= tile_width(h); }
 
 uint64_t low_mask(const zb_backend *h) {
     const uint32_t low = std::min<uint32_t>(h->prec == 64 ? 4 : 5, h->n);
@@ -1603,6 +1604,57 @@ int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms) {
         for (int r = 0; r < reps; ++r)
             hipLaunchKernelGGL(dev::k_copy, dim3(grid), dim3(dev::kBlock), 0, st, dst, src, n_doubles);
         check(hipGetLastError(), "k_copy");
+        check(hipEventRecord(e1, st), "hipEventRecord");
+        check(hipStreamSynchronize(st), "hipStreamSynchronize");
+        float f = 0.f;
+        check(hipEventElapsedTime(&f, e0, e1), "hipEventElapsedTime");
+        if (ms) *ms = reps > 0 ? (double)f / reps : 0.0;
+    });
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (st) (void)hipStreamDestroy(st);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return rc;
+}
+
+int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms) {
+    if (kind < 0 || kind > 3) {
+        set_error("zb_copy_probe: kind must be 0..3");
+        return ZB_ERR_ARG;
+    }
+    if (kind == 0) return zb_calibration_copy(n_doubles, reps, ms);
+    if (zb_device_count() == 0) {
+        set_error("no HIP device is available (the hip backend needs an AMD GPU)");
+        return ZB_ERR_UNSUPPORTED;
+    }
+    n_doubles &= ~uint64_t(1);  // whole 16-byte elements
+    double *src = nullptr, *dst = nullptr;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    const int rc = guarded([&] {
+        check(hipMalloc(&src, n_doubles * 8), "hipMalloc");
+        check(hipMalloc(&dst, n_doubles * 8), "hipMalloc");
+        check(hipStreamCreateWithFlags(&st, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&e0), "hipEventCreate");
+        check(hipEventCreate(&e1), "hipEventCreate");
+        const unsigned fill_grid = (unsigned)std::min<uint64_t>((n_doubles + dev::kBlock - 1) / dev::kBlock, 1u << 16);
+        hipLaunchKernelGGL(dev::k_fill, dim3(fill_grid), dim3(dev::kBlock), 0, st, src, n_doubles, 1.0);
+        const uint64_t n2 = n_doubles / 2;
+        const int u = kind == 1 ? 1 : 4;
+        const unsigned grid =
+            (unsigned)std::max<uint64_t>(1, std::min<uint64_t>((n2 / u + dev::kBlock - 1) / dev::kBlock, 1u << 16));
+        auto launch = [&] {
+            double2 *d = reinterpret_cast<double2 *>(dst);
+            const double2 *sv = reinterpret_cast<const double2 *>(src);
+            if (kind == 1) hipLaunchKernelGGL((dev::k_copy_wide<1, false>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
+            else if (kind == 2) hipLaunchKernelGGL((dev::k_copy_wide<4, false>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
+            else hipLaunchKernelGGL((dev::k_copy_wide<4, true>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
+        };
+        launch();  // warm
+        check(hipEventRecord(e0, st), "hipEventRecord");
+        for (int r = 0; r < reps; ++r) launch();
+        check(hipGetLastError(), "k_copy_wide");
         check(hipEventRecord(e1, st), "hipEventRecord");
         check(hipStreamSynchronize(st), "hipStreamSynchronize");
         float f = 0.f;

@@ -188,6 +188,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
     const int precision = zb_precision(hb);
     const zr_noise *nz = trivial(noise) ? nullptr : noise;
     std::unordered_map<uint64_t, uint64_t> raw;
+    std::vector<std::pair<uint64_t, uint64_t>> sampled;  // run_sampled's counts, ascending by word
     try {
         zb_backend *h = hb;
         ok(zb_set_mode(h, mode));
@@ -198,15 +199,33 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
             if (!c.final_measures.empty() && shots > 0) {
                 std::vector<uint64_t> smp(shots);
                 ok(zb_sample(h, shots, splitmix64(seed), smp.data()));
-                std::unordered_map<uint64_t, uint64_t> basis;
-                for (uint64_t s : smp) ++basis[s];
-                for (const auto &kv : basis) {
+                // Counts per basis index (exec.rs:437-441): the samples come
+                // back ascending, so equal indices are runs.  Then each unique
+                // index's clbit word (exec.rs:442-447); when the words come
+                // out strictly ascending (every qubit measured, clbits in
+                // qubit order) they are the result as they are, else they are
+                // sorted and equal words merged.
+                bool ascending = true;
+                for (size_t i = 0; i < smp.size();) {
+                    size_t j = i + 1;
+                    while (j < smp.size() && smp[j] == smp[i]) ++j;
                     uint64_t cl = 0;
                     for (const auto &fm : c.final_measures) {
-                        const uint64_t bit = (kv.first >> fm.first) & 1;
+                        const uint64_t bit = (smp[i] >> fm.first) & 1;
                         cl = (cl & ~(1ull << fm.second)) | (bit << fm.second);
                     }
-                    raw[cl] += kv.second;
+                    ascending = ascending && (sampled.empty() || sampled.back().first < cl);
+                    sampled.emplace_back(cl, (uint64_t)(j - i));
+                    i = j;
+                }
+                if (!ascending) {
+                    std::sort(sampled.begin(), sampled.end());
+                    size_t m = 0;
+                    for (size_t i = 0; i < sampled.size(); ++i) {
+                        if (m > 0 && sampled[m - 1].first == sampled[i].first) sampled[m - 1].second += sampled[i].second;
+                        else sampled[m++] = sampled[i];
+                    }
+                    sampled.resize(m);
                 }
             }
         } else {  // run_per_shot (exec.rs:459-549)
@@ -240,6 +259,10 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
         }
     } catch (const RunError &e) {
         return e.code;
+    }
+    if (!nz && !c.dynamic) {
+        out.swap(sampled);
+        return ZB_OK;
     }
     out.assign(raw.begin(), raw.end());
     std::sort(out.begin(), out.end());

@@ -902,6 +902,32 @@ static __global__ __launch_bounds__(kBlock) void k_copy(double *__restrict__ dst
     for (uint64_t i = gtid(); i < n; i += gstride()) dst[i] = src[i];
 }
 
+// Best-effort streaming copies, to measure what a read + write pass can
+// reach: 16-byte accesses, U of them in flight per lane before the stores,
+// and with NT the stores marked non-temporal (streamed past the caches).
+// n2 counts 16-byte elements; the grid covers them U at a time per lane.
+template <int U, bool NT>
+__global__ __launch_bounds__(kBlock) void k_copy_wide(double2 *__restrict__ dst, const double2 *__restrict__ src,
+                                                      uint64_t n2) {
+    const uint64_t stride = gstride();
+    uint64_t i = gtid();
+    for (; i + (U - 1) * stride < n2; i += U * stride) {
+        double2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = src[i + u * stride];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if constexpr (NT) {
+                __builtin_nontemporal_store(v[u].x, &dst[i + u * stride].x);
+                __builtin_nontemporal_store(v[u].y, &dst[i + u * stride].y);
+            } else {
+                dst[i + u * stride] = v[u];
+            }
+        }
+    }
+    for (; i < n2; i += stride) dst[i] = src[i];
+}
+
 static __global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ dst, uint64_t n, double v) {
     for (uint64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
 }

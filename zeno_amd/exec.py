@@ -143,11 +143,20 @@ class Counts(dict):
         return sorted(((k, v / t) for k, v in self.items()), key=lambda kv: (-kv[1], kv[0]))
 
 
+def counts_of(raw: dict, cregs) -> Counts:
+    """Counts keyed by format_key (src/exec.rs:235-253) from clbit words."""
+    out = Counts()
+    for k, v in raw.items():
+        key = format_key(k, cregs)
+        out[key] = out.get(key) + v
+    return out
+
+
 @dataclasses.dataclass
 class RunResult:
-    """exec::RunResult (src/exec.rs:115-130)."""
+    """exec::RunResult (src/exec.rs:115-130).  `counts` (bitstring keys) is
+    built from `raw_counts` (clbit words) on first use."""
 
-    counts: Counts
     raw_counts: dict
     shots: int
     n_qubits: int
@@ -159,6 +168,14 @@ class RunResult:
     stats: dict
     mem_bytes: int
     notices: list
+    cregs: list = dataclasses.field(default_factory=list)
+    _counts: Optional[Counts] = dataclasses.field(default=None, repr=False)
+
+    @property
+    def counts(self) -> Counts:
+        if self._counts is None:
+            self._counts = counts_of(self.raw_counts, self.cregs)
+        return self._counts
 
 
 class _MemPlan(C.Structure):
@@ -281,7 +298,7 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
             ptr(keys), ptr(counts), cap, C.byref(secs_c),
         )
         _check(m, "run")
-        raw = {int(k): int(v) for k, v in zip(keys[:m], counts[:m])}
+        raw = dict(zip(keys[:m].tolist(), counts[:m].tolist()))
         secs = secs_c.value
     if noise is not None:
         notices.append("noise: trajectory sampling — per-shot execution, fusion disabled")
@@ -290,14 +307,10 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
         # statevector cannot be asked for; run_sampled's otherwise (:426-433)
         notices.append("circuit has no measurements: counts are empty" if noise is not None else
                        "circuit has no measurements: counts are empty (use --statevector to inspect the state)")
-    counts = Counts()
-    for k, v in raw.items():
-        key = format_key(k, c.cregs)
-        counts[key] = counts.get(key) + v
     return RunResult(
-        counts=counts, raw_counts=raw, shots=opts.shots, n_qubits=c.n_qubits, precision=prec, backend=name,
+        raw_counts=raw, shots=opts.shots, n_qubits=c.n_qubits, precision=prec, backend=name,
         seed=seed, sim_time=secs, statevector=statevector, stats=c.stats,
-        mem_bytes=state_bytes(c.n_qubits, prec), notices=notices,
+        mem_bytes=state_bytes(c.n_qubits, prec), notices=notices, cregs=list(c.cregs),
     )
 
 
