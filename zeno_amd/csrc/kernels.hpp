@@ -14,11 +14,11 @@
 
 #include <cstdint>
 
+#include "devcommon.hpp"
+
 namespace zeno {
 namespace dev {
 
-constexpr int kBlock = 256;        // 4 wave64 per workgroup
-constexpr int kReduceSpan = 4096;  // elements per reduction block (16 per lane)
 constexpr int kMaxDiagK = 12;
 constexpr int kMaxFusedK = 6;
 
@@ -35,10 +35,6 @@ struct Qubits {
     uint32_t s[kMaxDiagK];
     int k;
 };
-
-__device__ __forceinline__ uint64_t insert_zero(uint64_t x, uint32_t q) {
-    return ((x >> q) << (q + 1)) | (x & ((1ull << q) - 1));
-}
 
 // Qubit layout: qubit q of the circuit is bit phys[q] of the state index in
 // HBM.  lut holds 5 tables of 256 entries, one per byte of a logical index,

@@ -19,7 +19,7 @@
 
 #include <cstdint>
 
-#include "kernels.hpp"
+#include "devcommon.hpp"
 
 namespace zeno {
 namespace dev {
