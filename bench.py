@@ -339,6 +339,77 @@ def bench_serve(args):
             "rates": results}
 
 
+def window_costs(args):
+    """Where a k_window pass's time goes, at the bench size: windows of
+    chosen composition on a state whose every amplitude is nonzero (a U3 on
+    each qubit first), each the median of 3 launches after one, and the
+    bench circuit's own windows with a least-squares fit of their duration
+    against their U3, CX, stage and diagonal counts."""
+    import numpy as np
+
+    import zeno_amd as z
+
+    n = args.n
+    rng = np.random.default_rng(1)
+
+    def u3():
+        q, _ = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))
+        return q
+
+    out = {"workload": f"{n} qubits, 12-bit tiles, 3 slots, default knobs", "controlled": {}, "circuit": {}}
+    prog = z.matching_circuit(n, args.depth, args.seed, rzz_period=5).to_program()[0]
+    comp = z.compile(prog, n, fusion=args.fusion)
+    for prec in (64, 32):
+        low = 4 if prec == 64 else 5
+        free = list(range(low, 12))
+        be = z.HipBackend(n, prec)
+        for q in range(n):
+            be.apply_unitary([q], u3())
+        be.finish()
+        cases = {
+            "3 U3 (1 stage)": lambda b: [b.apply_unitary([q], u3()) for q in free[:3]],
+            "6 U3 (2 stages)": lambda b: [b.apply_unitary([q], u3()) for q in free[:6]],
+            "12 U3 (4 stages)": lambda b: [b.apply_unitary([q], u3()) for q in range(12)],
+            "24 U3 (8 stages)": lambda b: [b.apply_unitary([q], u3()) for q in list(range(12)) * 2],
+            "4 CX (1 stage)": lambda b: [b.apply_cx(free[0], free[1]), b.apply_cx(free[2], free[0]),
+                                         b.apply_cx(free[1], free[2]), b.apply_cx(free[0], free[2])],
+            "2 diagonals on 10 qubits": lambda b: [b.apply_diagonal(list(range(2, 12)),
+                                                                    np.exp(1j * rng.uniform(0, 6.3, 1024)))
+                                                   for _ in range(2)],
+        }
+        for label, build in cases.items():
+            ts = []
+            for _ in range(4):
+                be.finish()
+                be.profile_begin()
+                build(be)
+                be.finish()
+                ts.append([ms for name, ms, _ in be.profile_end() if name == "k_window"][0])
+            out["controlled"][f"f{prec} {label}"] = round(float(np.median(ts[1:])), 3)
+            _log(f"window f{prec} {label}: {out['controlled'][f'f{prec} {label}']} ms")
+        be.reset_all()
+        be.apply_compiled(comp)
+        be.reset_all()
+        be.profile_begin()
+        be.apply_compiled(comp)
+        be.finish()
+        recs, det = be.profile_end(), be.profile_detail()
+        rows = [(ms, d) for (name, ms, _), d in zip(recs, det) if name == "k_window"]
+        X = np.array([[1, d[0] - d[2] - d[3], d[3], d[1], d[2]] for _, d in rows], float)
+        y = np.array([ms for ms, _ in rows])
+        coef, *_ = np.linalg.lstsq(X, y, rcond=None)
+        out["circuit"][f"f{prec}"] = {
+            "windows": len(rows), "mean_ms": round(float(y.mean()), 3), "total_ms": round(float(y.sum()), 1),
+            "fit_ms": {"base": round(coef[0], 3), "per_u3": round(coef[1], 3), "per_cx": round(coef[2], 3),
+                       "per_stage": round(coef[3], 3), "per_diagonal": round(coef[4], 3)},
+            "rms_residual_ms": round(float(np.sqrt(np.mean((y - X @ coef) ** 2))), 3),
+            "mean_counts": {"u3": round(float(X[:, 1].mean()), 2), "cx": round(float(X[:, 2].mean()), 2),
+                            "stages": round(float(X[:, 3].mean()), 2), "diagonals": round(float(X[:, 4].mean()), 2)}}
+        _log(f"circuit f{prec}: {out['circuit'][f'f{prec}']}")
+        be.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -366,6 +437,8 @@ def main():
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-pool", type=int, default=48, help="serving pool in GiB")
     ap.add_argument("--no-extra", action="store_true", help="skip the trajectory and serving legs")
+    ap.add_argument("--window-costs", action="store_true",
+                    help="only measure k_window passes of chosen composition and fit the circuit's windows")
     ap.add_argument("--no-small", action="store_true",
                     help="skip the 20-, 24- and 26-qubit end-to-end legs (GPU and CPU)")
     ap.add_argument("--no-variants", action="store_true",
@@ -387,6 +460,10 @@ def main():
     if z.device_count() == 0:
         print("bench.py needs a HIP device", file=sys.stderr)
         sys.exit(1)
+
+    if args.window_costs:
+        print(json.dumps(window_costs(args)), flush=True)
+        return
 
     n = args.n
     prog = z.matching_circuit(n, args.depth, args.seed, rzz_period=5).to_program()[0]

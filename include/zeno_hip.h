@@ -183,8 +183,10 @@ int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms);
  * calibration copy), average ms over `reps` after one untimed copy:
  * kind 0 = zb_calibration_copy (8-byte accesses); 1 = 16-byte accesses;
  * 2 = 16-byte accesses, four loads in flight per lane before the stores;
- * 3 = as 2 with non-temporal stores.  What a read + write pass reaches on
- * the device, beside the 8 TB/s of the data sheet. */
+ * 3 = as 2 with non-temporal stores; 4..7 = kinds 0..3 on data hashed
+ * from the index instead of a constant fill (values that change from
+ * element to element, as a state's do).  What a read + write pass reaches
+ * on the device, beside the 8 TB/s of the data sheet. */
 int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms);
 /* The backend's qubit layout: phys[q] = the bit of the state index that
  * holds circuit qubit q (n entries).  Every zb_* call takes and returns
@@ -238,6 +240,10 @@ int zb_tune_default(uint32_t n_qubits, int precision, const char *key,
 int zb_profile_begin(zb_backend *h);
 int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms,
                        double *bytes, int64_t cap);
+/* For the launches the last zb_profile_end returned, in the same order,
+ * 4 values each: a window's ops, stages, diagonal ops and CX ops (zero
+ * for other kernels).  Returns the number of launches written. */
+int64_t zb_profile_detail(const zb_backend *h, int32_t *detail, int64_t cap);
 
 /* ---------------------------------------------------------------------
  * Layer 2: compiler + executor (the host side above the boundary)

@@ -49,6 +49,7 @@ SIGNATURES = {
     "zb_tune": (I32, [P, C.c_char_p, I64]),
     "zb_tune_default": (I32, [U32, I32, C.c_char_p, I64]),
     "zb_profile_end": (I64, [P, P, P, P, I64]),
+    "zb_profile_detail": (I64, [P, P, I64]),
     "zr_compile": (P, [P, I64, U32, P, U32, I32, I32]),
     "zr_compiled_free": (None, [P]),
     "zr_compiled_nops": (I64, [P]),

@@ -208,6 +208,13 @@ class HipBackend:
         n = _check(lib().zb_profile_end(self._h, ptr(k), ptr(ms), ptr(by), cap), "profile_end")
         return [(_lib.KERNEL_NAMES.get(int(k[i]), str(int(k[i]))), float(ms[i]), float(by[i])) for i in range(n)]
 
+    def profile_detail(self, cap: int = 1 << 16):
+        """Per launch of the last profile_end: (ops, stages, diagonal ops,
+        CX ops) for windows, zeros for other kernels."""
+        d = np.zeros(4 * cap, np.int32)
+        n = int(lib().zb_profile_detail(self._h, ptr(d), cap))
+        return [tuple(int(x) for x in d[4 * i:4 * i + 4]) for i in range(n)]
+
     def apply_compiled(self, compiled, lo: int = 0, hi: int | None = None):
         hi = len(compiled) if hi is None else hi
         _check(lib().zr_apply_ops(self._h, compiled.handle, lo, hi), "apply_ops")

@@ -130,9 +130,10 @@ struct zb_backend {
         int kernel;
         double bytes;
         hipEvent_t a, b;
+        int32_t detail[4];  // windows: ops, stages, diagonal ops, CX ops (else zero)
     };
     bool prof = false;
-    std::vector<Rec> recs;
+    std::vector<Rec> recs, last_recs;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_next = 0;
     hipEvent_t next_event() {
@@ -144,10 +145,12 @@ struct zb_backend {
         return ev_pool[ev_next++];
     }
     // Brackets one launch with events when profiling (kernel ids in zeno_hip.h).
+    int32_t next_detail[4] = {0, 0, 0, 0};  // detail of the next timed launch
     template <class F>
     void timed(int kernel, double bytes, F &&launch) {
         if (!prof) return launch();
-        Rec r{kernel, bytes, next_event(), next_event()};
+        Rec r{kernel, bytes, next_event(), next_event(), {next_detail[0], next_detail[1], next_detail[2], next_detail[3]}};
+        for (int32_t &d : next_detail) d = 0;
         check(hipEventRecord(r.a, stream), "hipEventRecord");
         launch();
         check(hipEventRecord(r.b, stream), "hipEventRecord");
@@ -707,6 +710,14 @@ void flush_T(zb_backend *h) {
                         : static_cast<const T *>(h->upload(h->win_tables.data(), h->win_tables.size()));
     const uint64_t tiles = h->len >> m;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, h->win_grid);
+    if (h->prof) {
+        int32_t nd = 0, nc = 0;
+        for (const auto &p : h->win_ops) nd += p.kind == dev::kWDiag, nc += p.kind == dev::kWCx;
+        h->next_detail[0] = (int32_t)h->win_ops.size();
+        h->next_detail[1] = (int32_t)stages.size();
+        h->next_detail[2] = nd;
+        h->next_detail[3] = nc;
+    }
     h->timed(ZB_K_WINDOW, h->sweep_bytes(), [&] {
         launch_window<T>(h, ns, dense, grid, dblob, dtab, tiles);
     });
@@ -1619,11 +1630,13 @@ int zb_calibration_copy(uint64_t n_doubles, int reps, double *ms) {
 }
 
 int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms) {
-    if (kind < 0 || kind > 3) {
-        set_error("zb_copy_probe: kind must be 0..3");
+    if (kind < 0 || kind > 7) {
+        set_error("zb_copy_probe: kind must be 0..7");
         return ZB_ERR_ARG;
     }
     if (kind == 0) return zb_calibration_copy(n_doubles, reps, ms);
+    const bool hashed = kind >= 4;  // 4..7: kinds 0..3 on hashed, not constant, data
+    if (hashed) kind -= 4;
     if (zb_device_count() == 0) {
         set_error("no HIP device is available (the hip backend needs an AMD GPU)");
         return ZB_ERR_UNSUPPORTED;
@@ -1639,7 +1652,8 @@ int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms) {
         check(hipEventCreate(&e0), "hipEventCreate");
         check(hipEventCreate(&e1), "hipEventCreate");
         const unsigned fill_grid = (unsigned)std::min<uint64_t>((n_doubles + dev::kBlock - 1) / dev::kBlock, 1u << 16);
-        hipLaunchKernelGGL(dev::k_fill, dim3(fill_grid), dim3(dev::kBlock), 0, st, src, n_doubles, 1.0);
+        if (hashed) hipLaunchKernelGGL(dev::k_fill_hash, dim3(fill_grid), dim3(dev::kBlock), 0, st, src, n_doubles);
+        else hipLaunchKernelGGL(dev::k_fill, dim3(fill_grid), dim3(dev::kBlock), 0, st, src, n_doubles, 1.0);
         const uint64_t n2 = n_doubles / 2;
         const int u = kind == 1 ? 1 : 4;
         const unsigned grid =
@@ -1647,7 +1661,9 @@ int zb_copy_probe(uint64_t n_doubles, int reps, int kind, double *ms) {
         auto launch = [&] {
             double2 *d = reinterpret_cast<double2 *>(dst);
             const double2 *sv = reinterpret_cast<const double2 *>(src);
-            if (kind == 1) hipLaunchKernelGGL((dev::k_copy_wide<1, false>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
+            if (kind == 0)
+                hipLaunchKernelGGL(dev::k_copy, dim3(fill_grid), dim3(dev::kBlock), 0, st, dst, src, n_doubles);
+            else if (kind == 1) hipLaunchKernelGGL((dev::k_copy_wide<1, false>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
             else if (kind == 2) hipLaunchKernelGGL((dev::k_copy_wide<4, false>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
             else hipLaunchKernelGGL((dev::k_copy_wide<4, true>), dim3(grid), dim3(dev::kBlock), 0, st, d, sv, n2);
         };
@@ -1701,6 +1717,16 @@ int zb_profile_begin(zb_backend *h) {
     });
 }
 
+int64_t zb_profile_detail(const zb_backend *h, int32_t *detail, int64_t cap) {
+    int64_t n = 0;
+    for (const auto &r : h->last_recs) {
+        if (n >= cap) break;
+        for (int i = 0; i < 4; ++i) detail[4 * n + i] = r.detail[i];
+        ++n;
+    }
+    return n;
+}
+
 int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms, double *bytes, int64_t cap) {
     int64_t n = 0;
     const int rc = guarded([&] {
@@ -1716,6 +1742,7 @@ int64_t zb_profile_end(zb_backend *h, int32_t *kernel, double *ms, double *bytes
             bytes[n] = r.bytes;
             ++n;
         }
+        h->last_recs.assign(h->recs.begin(), h->recs.begin() + n);
         h->recs.clear();
         h->ev_next = 0;
     });

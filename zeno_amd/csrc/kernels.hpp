@@ -928,6 +928,19 @@ static __global__ __launch_bounds__(kBlock) void k_fill(double *__restrict__ dst
     for (uint64_t i = gtid(); i < n; i += gstride()) dst[i] = v;
 }
 
+// dst[i] = a uniform double in [-1, 1) hashed from i (splitmix64 finaliser):
+// copy probes on data whose bits change from element to element, as a
+// state's do.
+static __global__ __launch_bounds__(kBlock) void k_fill_hash(double *__restrict__ dst, uint64_t n) {
+    for (uint64_t i = gtid(); i < n; i += gstride()) {
+        uint64_t z = i + 0x9E3779B97F4A7C15ull;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        dst[i] = (double)(z >> 11) * 0x1.0p-52 - 1.0;
+    }
+}
+
 // max_i |a_i - b_i| (complex modulus, f64) between two states of possibly
 // different precision; non-negative doubles order like their bit patterns,
 // so the grid combines block maxima with an integer atomicMax.
