@@ -342,9 +342,10 @@ def bench_serve(args):
 def window_costs(args):
     """Where a k_window pass's time goes, at the bench size: windows of
     chosen composition on a state whose every amplitude is nonzero (a U3 on
-    each qubit first), each the median of 3 launches after one, and the
-    bench circuit's own windows with a least-squares fit of their duration
-    against their U3, CX, stage and diagonal counts."""
+    each qubit first) and one on the all-zero state, each the median of 3
+    launches after one, and the bench circuit's own windows with a
+    least-squares fit of their duration against their U3, CX, stage and
+    diagonal counts."""
     import numpy as np
 
     import zeno_amd as z
@@ -363,6 +364,23 @@ def window_costs(args):
         low = 4 if prec == 64 else 5
         free = list(range(low, 12))
         be = z.HipBackend(n, prec)
+
+        def timed(build):
+            ts = []
+            for _ in range(4):
+                be.finish()
+                be.profile_begin()
+                build(be)
+                be.finish()
+                ts.append([ms for name, ms, _ in be.profile_end() if name == "k_window"][0])
+            return round(float(np.median(ts[1:])), 3)
+
+        # the same window on |0...0>: every tile but the first stays zero, so
+        # the instructions and bytes are those of a random state and only the
+        # switching load differs
+        label = f"f{prec} 12 U3 (4 stages), all-zero state"
+        out["controlled"][label] = timed(lambda b: [b.apply_unitary([q], u3()) for q in range(12)])
+        _log(f"window {label}: {out['controlled'][label]} ms")
         for q in range(n):
             be.apply_unitary([q], u3())
         be.finish()
@@ -378,14 +396,7 @@ def window_costs(args):
                                                    for _ in range(2)],
         }
         for label, build in cases.items():
-            ts = []
-            for _ in range(4):
-                be.finish()
-                be.profile_begin()
-                build(be)
-                be.finish()
-                ts.append([ms for name, ms, _ in be.profile_end() if name == "k_window"][0])
-            out["controlled"][f"f{prec} {label}"] = round(float(np.median(ts[1:])), 3)
+            out["controlled"][f"f{prec} {label}"] = timed(build)
             _log(f"window f{prec} {label}: {out['controlled'][f'f{prec} {label}']} ms")
         be.reset_all()
         be.apply_compiled(comp)
