@@ -333,7 +333,10 @@ int zr_resolve_precision(uint32_t n_qubits, int requested, uint64_t budget,
  * run per shot (run_per_shot).  Writes the unique clbit words with their
  * counts, ascending by word; returns how many, or < 0.  sim_seconds covers
  * state allocation, every op, finish and sampling (src/exec.rs:372-381).
- * mode: see zb_set_mode. */
+ * mode: 0 and 1 as zb_set_mode; per-shot circuits in mode 1 run as batched
+ * trajectories (zr_run_trajectories) where that is the faster way, else
+ * the host's per-shot loop; mode 2 is mode 1 with the per-shot loop always
+ * (the words are the same either way). */
 int64_t zr_run(const zr_compiled *c, int precision, uint64_t shots,
                uint64_t seed, const zr_noise *noise, int mode,
                uint64_t *keys, uint64_t *counts, int64_t cap,
@@ -344,8 +347,8 @@ int64_t zr_run(const zr_compiled *c, int precision, uint64_t shots,
  * captured in a hipGraph, each workgroup executing whole trajectories on
  * its own state.  Shot s uses Xoshiro256++ seeded with
  * splitmix64(seed ^ s) and the frozen draw order, so out[s] (the shot's
- * clbit word) equals the CPU per-shot path's.  Needs n <= 20 and
- * unitary/diagonal ops on <= 3 qubits (zr_trajectories_supported);
+ * clbit word) equals the CPU per-shot path's.  Needs n <= 26, dense ops on
+ * <= 6 qubits and diagonals on <= 12 (zr_trajectories_supported);
  * kernel_seconds gets the launch-to-completion time.  The call allocates
  * one workspace for up to two concurrent trajectories per CU, fewer when
  * free device memory is short (ZB_ERR_MEMORY when not even one fits). */

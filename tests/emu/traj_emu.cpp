@@ -7,6 +7,8 @@
 //
 //   traj_emu PROG N CREGS(csv) FUSION PRECISION SHOTS SEED NOISE(7 csv) ORDER OUT
 //
+// Built with -DEMU_PREC=32 or 64; PRECISION must match.
+//
 // ORDER: lane order in every barrier phase (0 forward, 1 backward, >= 2 a
 // seeded shuffle).
 //
@@ -15,6 +17,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "lanes.hpp"
@@ -41,8 +44,8 @@ template <typename T>
 int run(const Compiled &c, uint64_t shots, uint64_t seed, const std::vector<double> &nzv, int order,
         const char *out_path) {
     const Tables<T> t = pack<T>(c);
-    const uint64_t len = 1ull << c.n_qubits;
-    std::vector<T> states(2 * len);  // one workgroup: one state
+    const uint64_t slot = traj_slot_bytes(c.n_qubits, sizeof(T)) / sizeof(T);
+    std::vector<T> states(slot);  // one workgroup: one slot (state, block sums)
     std::vector<uint64_t> out(shots ? shots : 1, ~0ull);
     dev::TrajLaunch<T> L{};
     L.states = states.data();
@@ -52,6 +55,7 @@ int run(const Compiled &c, uint64_t shots, uint64_t seed, const std::vector<doub
     L.out = out.data();
     L.seed = seed;
     L.shots = shots;
+    L.slot = slot;
     L.n = c.n_qubits;
     L.nops = (uint32_t)t.nops;
     L.nfm = (uint32_t)t.nfm;
@@ -61,7 +65,8 @@ int run(const Compiled &c, uint64_t shots, uint64_t seed, const std::vector<doub
     emu::block_idx.x = 0;
     emu::grid_dim.x = 1;
     emu::block_dim.x = dev::kBlock;
-    emu::run_group(dev::kBlock, order, [&] { dev::k_trajectories<T>(&L); });
+    if (traj_wide(c)) emu::run_group(dev::kBlock, order, [&] { dev::k_trajectories<T, dev::kTrajWideDense>(&L); });
+    else emu::run_group(dev::kBlock, order, [&] { dev::k_trajectories<T, dev::kTrajMaxK>(&L); });
     FILE *f = std::fopen(out_path, "wb");
     if (!f) return 2;
     std::fwrite(out.data(), 8, shots, f);
@@ -90,6 +95,11 @@ int main(int argc, char **argv) {
     const std::vector<double> nz = csv(argv[8]);
     if (nz.size() != 7) return 2;
     const Compiled c = compile(prog.data(), (int64_t)prog.size(), n, cregs.data(), (uint32_t)cregs.size(), fusion, 10);
+    if (!traj_fits(c)) return 3;
     const int order = std::atoi(argv[9]);
-    return prec == 64 ? run<double>(c, shots, seed, nz, order, argv[10]) : run<float>(c, shots, seed, nz, order, argv[10]);
+    // built for one precision (EMU_PREC), so that each build instantiates
+    // half the kernels
+    if (prec != EMU_PREC) return 4;
+    using T = std::conditional_t<EMU_PREC == 64, double, float>;
+    return run<T>(c, shots, seed, nz, order, argv[10]);
 }

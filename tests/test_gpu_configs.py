@@ -7,9 +7,9 @@ smaller parity tests do not reach, through the C ABI.
   can check); f32 sweeps and fused 5-qubit blocks on the matrix cores are
   held to stated bounds against it, and f64 to its norm and to the adjoint
   returning |0...0>.
-- The per-shot path executed one shot at a time in tiled mode (circuits the
-  batched kernel does not take: fused blocks of more than 3 qubits, more
-  than 20 qubits), against the oracle shot for shot.
+- The per-shot path executed one shot at a time in tiled mode, and the
+  batched launch above 20 qubits and with fused blocks of more than 3
+  qubits, against the oracle shot for shot.
 - The HBM precision policy forcing f32, re-bound handles carrying no knob
   of an earlier shape, noisy requests served from the pool, and all 4096
   trajectories of the noisy-trajectory configuration against fixtures the
@@ -120,10 +120,11 @@ def _counts(words):
 @pytest.mark.parametrize("prec", [64, 32])
 def test_per_shot_loop_in_tiled_mode_with_fused_blocks(orc, prec):
     """A 13-qubit dynamic circuit (mid-circuit measure, reset, `if`)
-    compiled at fusion 4, so the batched kernel does not take it: every
-    shot runs through zb_* calls in tiled mode — windows, held-back CX ops,
-    and the prob_one / measure flushes between them — and the counts equal
-    the oracle's."""
+    compiled at fusion 4.  With the per-shot loop on the host (zr_run mode
+    2) every shot runs through zb_* calls in tiled mode — windows, held-back
+    CX ops, and the prob_one / measure flushes between them — and the counts
+    equal the oracle's; the batched launch (its wide instance: 4-qubit
+    blocks) gives the oracle's word for every shot."""
     n = 13
     I = orc.instr
     rows = [orc.gen_random(n, 4, 41), orc.measure(3, 0), orc.reset(5), I("h", [5]),
@@ -133,26 +134,71 @@ def test_per_shot_loop_in_tiled_mode_with_fused_blocks(orc, prec):
     prog = orc.program(rows)
     cregs = [2, n]
     zc = z.compile(prog, n, cregs, fusion=4)
-    assert zc.dynamic and not z.trajectories_supported(zc)
-    want = _counts(_shot_words(orc, prog, n, cregs, 4, prec, 12, 21, None))
+    assert zc.dynamic and z.trajectories_supported(zc)
+    assert max(len(o["qubits"]) for o in zc.ops() if o["kind"] == z.exec.OP_U) == 4
+    words = _shot_words(orc, prog, n, cregs, 4, prec, 12, 21, None)
     _say(f"per-shot tiled 13q dynamic fusion 4 prec={prec}")
-    got = z.run_compiled(zc, z.RunOptions(shots=12, seed=21, precision=prec, mode=z.HipBackend.TILED))
-    assert got.raw_counts == want
+    got = z.run_compiled(zc, z.RunOptions(shots=12, seed=21, precision=prec, mode=2))
+    assert got.raw_counts == _counts(words)
+    _say(f"batched 13q dynamic fusion 4 prec={prec}")
+    bw, _ = z.run_trajectories(zc, 12, 21, prec)
+    assert np.array_equal(bw, words)
 
 
-def test_per_shot_loop_above_the_batched_size(orc):
-    """A 21-qubit noisy circuit (above the batched kernel's 20 qubits):
-    the shots run one by one in tiled mode and equal the oracle's."""
+def test_per_shot_loop_above_20_qubits(orc):
+    """A 21-qubit noisy circuit with 4 shots: fewer concurrent states than
+    CUs, so zr_run keeps the per-shot loop (mode 1) as mode 2 does, and the
+    counts equal the oracle's."""
     n = 21
     nz = {"depolarizing_1q": 0.02, "depolarizing_2q": 0.05, "amplitude_damping": 0.03,
           "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
     prog = orc.program([orc.gen_random(n, 2, 210), orc.measure_all(n)])
     zc = z.compile(prog, n, fusion=0)
-    assert not z.trajectories_supported(zc)
     want = _counts(_shot_words(orc, prog, n, None, 0, 32, 4, 8, nz, threads=4))
-    _say("per-shot tiled 21q noisy f32")
-    got = z.run_compiled(zc, z.RunOptions(shots=4, seed=8, precision=32, noise=z.NoiseModel(**nz)))
-    assert got.raw_counts == want
+    for mode in (1, 2):
+        _say(f"per-shot tiled 21q noisy f32 mode={mode}")
+        got = z.run_compiled(zc, z.RunOptions(shots=4, seed=8, precision=32, noise=z.NoiseModel(**nz), mode=mode))
+        assert got.raw_counts == want
+
+
+@pytest.mark.parametrize("n,prec", [(22, 32), (22, 64), (24, 32)])
+def test_batched_trajectories_above_20_qubits(orc, n, prec):
+    """The batched launch above 20 qubits, where every probability sum has
+    more than 256 blocks (their totals reduced level by level in each
+    workgroup's slot): a noisy circuit at fusion 0 and a dynamic one at
+    fusion 5 (the wide instance: dense blocks on up to 5 qubits), 16 shots
+    each, every shot's word equal to the oracle's."""
+    I = orc.instr
+    nz = {"depolarizing_1q": 0.02, "depolarizing_2q": 0.05, "amplitude_damping": 0.05, "bit_flip": 0.01,
+          "readout_flip_0to1": 0.02, "readout_flip_1to0": 0.02}
+    noisy = orc.program([orc.gen_random(n, 1, 300 + n), I("ccx", [0, n // 2, n - 1]), orc.measure_all(n)])
+    dyn = orc.program([orc.gen_random(n, 3, 310 + n), orc.measure(1, 0), orc.reset(n - 2),
+                       I("rx", [n - 3], [0.7], cond=(0, 1)), orc.gen_random(n, 2, 320 + n)]
+                      + [orc.measure(q, 1 + q) for q in range(n)])
+    for label, prog, cregs, fusion, noise in (("noisy", noisy, None, 0, nz), ("dynamic", dyn, [1, n], 5, None)):
+        zc = z.compile(prog, n, cregs, fusion=fusion)
+        assert z.trajectories_supported(zc)
+        _say(f"batched {n}q {label} fusion {fusion} prec={prec}")
+        want = _shot_words(orc, prog, n, cregs, fusion, prec, 16, 77, noise, threads=16)
+        got, secs = z.run_trajectories(zc, 16, 77, prec, z.NoiseModel(**noise) if noise else None)
+        _say(f"  {secs:.3f} s")
+        assert np.array_equal(got, want), np.nonzero(got != want)[0][:8]
+
+
+def test_served_mode_choice_above_20_qubits(orc):
+    """512 noisy 22-qubit shots: zr_run takes the batched launch (one
+    trajectory per CU and more) in mode 1 and the per-shot loop in mode 2;
+    both give the oracle's counts."""
+    n, shots = 22, 512
+    nz = {"amplitude_damping": 0.02, "readout_flip_0to1": 0.03}
+    prog = orc.program([orc.gen_random(n, 1, 400), orc.measure_all(n)])
+    zc = z.compile(prog, n, fusion=0)
+    want = _counts(_shot_words(orc, prog, n, None, 0, 32, shots, 9, nz, threads=16))
+    for mode in (1, 2):
+        _say(f"zr_run 22q noisy f32 {shots} shots mode={mode}")
+        got = z.run_compiled(zc, z.RunOptions(shots=shots, seed=9, precision=32, noise=z.NoiseModel(**nz), mode=mode))
+        _say(f"  {got.sim_time:.3f} s")
+        assert got.raw_counts == want
 
 
 def test_auto_precision_falls_back_to_f32_on_the_hbm_budget(orc):

@@ -191,7 +191,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
     std::vector<std::pair<uint64_t, uint64_t>> sampled;  // run_sampled's counts, ascending by word
     try {
         zb_backend *h = hb;
-        ok(zb_set_mode(h, mode));
+        ok(zb_set_mode(h, mode == 2 ? 1 : mode));  // 2: tiled, per-shot loop on the host
         if (!state_is_zero) ok(zb_reset_all(h));
         if (!nz && !c.dynamic) {  // run_sampled (exec.rs:407-452)
             apply_ops_ahead(h, c.ops, 0, c.ops.size());
@@ -230,7 +230,15 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
             }
         } else {  // run_per_shot (exec.rs:459-549)
             bool batched = false;
-            if ((c.dynamic || !c.final_measures.empty()) && mode != 0 && trajectories_supported(c)) {
+            uint64_t avail = 0;  // the workspace the batched launch would get
+            if (traj) {
+                avail = traj->bytes;
+            } else {
+                uint64_t fr = 0, tot = 0;
+                if (zb_mem_info(&fr, &tot) == ZB_OK) avail = fr > kTrajMargin ? fr - kTrajMargin : 0;
+            }
+            if ((c.dynamic || !c.final_measures.empty()) && mode == 1 &&
+                traj_preferred(c, precision, shots, avail)) {
                 // every shot in one batched trajectory launch; when no
                 // workspace fits, the shots run one by one below
                 std::vector<uint64_t> words(shots);

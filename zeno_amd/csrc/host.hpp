@@ -131,6 +131,8 @@ struct TrajWorkspace {
     TrajGraph *graph = nullptr;
 };
 constexpr uint64_t kTrajLaunchBlockBytes = 256;
+// Free device memory a standalone batched run leaves to everything else.
+constexpr uint64_t kTrajMargin = 256ull << 20;
 // Concurrent trajectories worth running: min(shots, 2 per CU), at least 1.
 uint64_t traj_max_grid(uint64_t shots);
 // Workspace bytes for `grid` concurrent trajectories of c.
@@ -142,6 +144,12 @@ int run_trajectories_in(const Compiled &c, int precision, uint64_t shots, uint64
 void traj_graph_free(TrajGraph *g);
 uint64_t traj_graph_replays(const TrajGraph *g);
 bool trajectories_supported(const Compiled &c);
+// Whether the batched launch is the faster way to run c's shots with a
+// workspace of `bytes`: always up to 20 qubits (a trajectory's state stays
+// in the L2 / last-level cache); above, where every sweep streams HBM, only
+// with at least one concurrent trajectory per CU, since fewer stream less
+// than the chip's bandwidth that the serial loop's full-grid sweeps use.
+bool traj_preferred(const Compiled &c, int precision, uint64_t shots, uint64_t bytes);
 
 // Small static circuits as instances of one persistent launch (batch.hip).
 // A runner owns a stream, a device workspace with its pinned staging, a

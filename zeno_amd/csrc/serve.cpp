@@ -359,7 +359,9 @@ int64_t zs_submit(zs_server *s, const zr_compiled *c, int precision, uint64_t sh
         uint64_t grid = std::min<uint64_t>(traj_max_grid(shots), std::max<uint64_t>(1, kTrajServeBytes / bytes));
         while (grid > 1 && traj_workspace_bytes(cc, precision, shots, grid) > s->pool_bytes) grid /= 2;
         const uint64_t ws = traj_workspace_bytes(cc, precision, shots, grid);
-        if (ws <= s->pool_bytes) bytes = std::max(bytes, ws);
+        // a workspace the batched launch would not use (traj_preferred) is
+        // not charged: the request then runs the per-shot loop in its state
+        if (ws <= s->pool_bytes && traj_preferred(cc, precision, shots, ws)) bytes = std::max(bytes, ws);
     }
     auto r = std::make_unique<Request>();
     r->circuit.c = c->c;  // the request keeps its own copy of the op stream

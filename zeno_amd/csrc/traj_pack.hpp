@@ -9,6 +9,35 @@
 
 namespace zeno {
 
+// Circuits the batched kernel runs: up to kTrajMaxQubits qubits, dense ops
+// on up to 6 qubits, diagonals on up to 12.
+inline bool traj_fits(const Compiled &c) {
+    if (c.n_qubits > (uint32_t)dev::kTrajMaxQubits) return false;
+    for (const Op &o : c.ops) {
+        if (o.kind == kUnitary && o.qs.size() > (size_t)dev::kTrajWideDense) return false;
+        if (o.kind == kDiagonal && o.qs.size() > (size_t)dev::kTrajOpQubits) return false;
+    }
+    return true;
+}
+
+// Whether the circuit needs the wide kernel instance (an op on more than 3
+// qubits).
+inline bool traj_wide(const Compiled &c) {
+    for (const Op &o : c.ops)
+        if ((o.kind == kUnitary || o.kind == kDiagonal) && o.qs.size() > (size_t)dev::kTrajMaxK) return true;
+    return false;
+}
+
+// Bytes of one workgroup's slot: re and im of 2^n values of `elem` bytes,
+// then, when the probability sums have more than 256 blocks, the block
+// sums and the next level's totals (f64), rounded up to 256 bytes.
+inline uint64_t traj_slot_bytes(uint32_t n, uint64_t elem) {
+    const uint64_t len = 1ull << n, nb = (len >> 1) / dev::kReduceSpan;
+    uint64_t b = 2 * len * elem;
+    if (nb > (uint64_t)dev::kTrajLdsSumBlocks) b += (nb + (nb + dev::kReduceSpan - 1) / dev::kReduceSpan + 1) * 8;
+    return (b + 255) & ~uint64_t(255);
+}
+
 // The op tables of one run in the kernel's format: ops, matrices in T,
 // final-measure pairs.
 template <typename T>

@@ -27,13 +27,7 @@ void hcheck(hipError_t e, const char *what) {
     if (e != hipSuccess) throw TrajError{ZB_ERR_DEVICE, std::string(what) + ": " + hipGetErrorString(e)};
 }
 
-bool supported(const Compiled &c) {
-    if (c.n_qubits > 20) return false;
-    for (const Op &o : c.ops) {
-        if ((o.kind == kUnitary || o.kind == kDiagonal) && o.qs.size() > (size_t)dev::kTrajMaxK) return false;
-    }
-    return true;
-}
+bool supported(const Compiled &c) { return traj_fits(c); }
 
 constexpr uint64_t kAlign = 256;
 uint64_t up(uint64_t x) { return (x + kAlign - 1) & ~(kAlign - 1); }
@@ -52,7 +46,7 @@ uint64_t tables_bytes(const Compiled &c, uint64_t shots) {
 }
 
 uint64_t state_bytes_of(const Compiled &c, int precision) {
-    return up(2 * (1ull << c.n_qubits) * (precision == 64 ? 8 : 4));
+    return traj_slot_bytes(c.n_qubits, precision == 64 ? 8 : 4);
 }
 
 uint64_t cu_count() {
@@ -71,13 +65,15 @@ uint64_t cu_count() {
 namespace zeno {
 
 // A captured launch: replayed as long as the launch block's address, the
-// grid and the precision repeat (the block's address is its only argument).
+// grid, the precision and the kernel instance repeat (the block's address
+// is its only argument).
 struct TrajGraph {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     void *block = nullptr;
     uint64_t grid = 0;
     int precision = 0;
+    bool wide = false;
     uint64_t replays = 0;
 };
 
@@ -103,6 +99,12 @@ uint64_t traj_grid_for(const Compiled &c, int precision, uint64_t shots, uint64_
     const uint64_t tb = precision == 64 ? tables_bytes<double>(c, shots) : tables_bytes<float>(c, shots);
     if (bytes < tb) return 0;
     return std::min<uint64_t>(traj_max_grid(shots), (bytes - tb) / state_bytes_of(c, precision));
+}
+
+bool traj_preferred(const Compiled &c, int precision, uint64_t shots, uint64_t bytes) {
+    if (!supported(c)) return false;
+    if (c.n_qubits <= 20) return true;
+    return traj_grid_for(c, precision, shots, bytes) >= cu_count();
 }
 
 }  // namespace zeno
@@ -142,6 +144,7 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
     L.out = dout;
     L.seed = seed;
     L.shots = shots;
+    L.slot = state_bytes_of(c, precision) / sizeof(T);
     L.n = c.n_qubits;
     L.nops = (uint32_t)t.nops;
     L.nfm = (uint32_t)t.nfm;
@@ -161,9 +164,10 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
     hcheck(hipStreamSynchronize(stream), "hipStreamSynchronize");
     const auto t0 = std::chrono::steady_clock::now();
     // One persistent launch, captured once per (launch block, grid,
-    // precision) and replayed while those repeat.
+    // precision, instance) and replayed while those repeat.
+    const bool wide = traj_wide(c);
     TrajGraph *&g = w.graph;
-    if (g && (g->block != (void *)dl || g->grid != grid || g->precision != precision)) {
+    if (g && (g->block != (void *)dl || g->grid != grid || g->precision != precision || g->wide != wide)) {
         traj_graph_free(g);
         g = nullptr;
     }
@@ -172,10 +176,15 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
         ng->block = dl;
         ng->grid = grid;
         ng->precision = precision;
+        ng->wide = wide;
         try {
             hcheck(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            hipLaunchKernelGGL(dev::k_trajectories<T>, dim3((unsigned)grid), dim3(dev::kBlock), 0, stream,
-                               (const dev::TrajLaunch<T> *)dl);
+            if (wide)
+                hipLaunchKernelGGL((dev::k_trajectories<T, dev::kTrajWideDense>), dim3((unsigned)grid),
+                                   dim3(dev::kBlock), 0, stream, (const dev::TrajLaunch<T> *)dl);
+            else
+                hipLaunchKernelGGL((dev::k_trajectories<T, dev::kTrajMaxK>), dim3((unsigned)grid), dim3(dev::kBlock),
+                                   0, stream, (const dev::TrajLaunch<T> *)dl);
             const hipError_t le = hipGetLastError();
             const hipError_t ce = hipStreamEndCapture(stream, &ng->graph);
             hcheck(le, "k_trajectories");
@@ -200,7 +209,7 @@ void run_T(const Compiled &c, uint64_t shots, uint64_t seed, const zr_noise *noi
 
 int checked_args(const Compiled &c, int precision) {
     if (!supported(c)) {
-        set_error("batched trajectories need <= 20 qubits and unitary/diagonal ops on <= 3 qubits");
+        set_error("batched trajectories need <= 26 qubits, dense ops on <= 6 qubits and diagonals on <= 12");
         return ZB_ERR_UNSUPPORTED;
     }
     if (precision != 32 && precision != 64) {
@@ -246,8 +255,7 @@ extern "C" int zr_run_trajectories(const zr_compiled *c, int precision, uint64_t
         set_error("hipMemGetInfo failed");
         return ZB_ERR_DEVICE;
     }
-    const uint64_t margin = 256ull << 20;
-    const uint64_t avail = free_b > margin ? free_b - margin : 0;
+    const uint64_t avail = free_b > kTrajMargin ? free_b - kTrajMargin : 0;
     const uint64_t grid = traj_grid_for(c->c, precision, shots, avail);
     if (grid == 0) {
         set_error("no trajectory workspace of " + std::to_string(c->c.n_qubits) + " qubits fits in " +

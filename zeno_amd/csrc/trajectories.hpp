@@ -13,6 +13,13 @@
 // Every helper is force-inlined: the kernel then makes no device-function
 // calls and needs no scratch.  No-jump damping steps are folded into the
 // next sweep (Pend below), which removes about half of the state passes.
+//
+// Two instances: KM = 3 takes unitaries and diagonals on <= 3 qubits (every
+// noisy circuit: the reference compiles those without fusion, src/lib.rs:
+// 226-233); KM = 6 also takes fused dense blocks on up to 6 qubits and
+// diagonals on up to 12, with a larger LDS matrix stage.  States of more
+// than 21 qubits keep their probability sums' block totals in a global
+// scratch area after the state (more than 256 blocks).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -24,12 +31,16 @@
 namespace zeno {
 namespace dev {
 
-constexpr int kTrajMaxK = 3;
+constexpr int kTrajMaxK = 3;        // damping scales held at once; the narrow instance's op width
+constexpr int kTrajWideDense = 6;   // the wide instance: dense blocks on up to 6 qubits
+constexpr int kTrajOpQubits = 12;   // ... and diagonals on up to 12
+constexpr int kTrajMaxQubits = 26;  // largest trajectory state
+constexpr int kTrajLdsSumBlocks = 256;  // block totals kept in LDS (n <= 21)
 
 struct TrajOp {
     uint32_t kind;  // 0 unitary, 1 diagonal, 2 cx, 3 measure, 4 reset
-    uint32_t k;     // qubits of a unitary/diagonal (1..3)
-    uint32_t q[kTrajMaxK];  // sorted qubits; cx: control, target; measure/reset: qubit
+    uint32_t k;     // qubits of a unitary/diagonal
+    uint32_t q[kTrajOpQubits];  // sorted qubits; cx: control, target; measure/reset: qubit
     uint32_t clbit;
     uint32_t is_if, creg_off, creg_len;
     uint32_t data_off;  // matrix (2*4^k T) or diagonal (2*2^k T) in the data block
@@ -83,7 +94,14 @@ struct TrajCtx {
     uint64_t len;
     double *red;     // LDS: 8 x 256 lane partials + 256 block sums
     const T *pauli;  // LDS: X then Y as [re 4 | im 4]
+    double *gsum;    // global: the block sums when there are more than 256, then the next level's
 };
+
+// Where block b's sum goes: LDS while the blocks fit it, else global.
+template <typename T>
+__device__ __forceinline__ double *block_sums(const TrajCtx<T> &c) {
+    return c.gsum ? c.gsum : c.red + 8 * kBlock;
+}
 
 // Pending damping scales.  A no-jump amplitude-damping step is the
 // diagonal diag(sc, sc*sqrt(1-gamma)) on one qubit (noise.rs:114-127).
@@ -127,15 +145,16 @@ __device__ __forceinline__ void t_pend_flush(const TrajCtx<T> c, Pend<T> &p) {
 
 // prob_one in the backend's fixed order: 4096-element blocks (16 serial
 // adds per lane, pairwise tree over 256 lanes), then the block sums the
-// same way.  Needs blockDim.x == 256 and len/2 <= 2^20 (n <= 21).  Pending
-// damping scales are applied to each loaded amplitude, not stored.
+// same way, level by level (k_reduce_f64's order).  Needs blockDim.x ==
+// 256.  Pending damping scales are applied to each loaded amplitude, not
+// stored.
 //
 // t_prob_blocks: the lane partials of blocks b0 .. b0+nchunk-1 (nchunk <= 8)
 // are in red[bi * 256 + lane]; each block's pairwise tree goes to bsum.
 template <typename T>
 __device__ __forceinline__ void t_prob_blocks(const TrajCtx<T> c, int b0, int nchunk) {
     double *s = c.red;
-    double *bsum = c.red + 8 * kBlock;
+    double *bsum = block_sums(c);
     const int t = threadIdx.x;
     __syncthreads();
     for (int st = kBlock / 2; st >= 1; st >>= 1) {
@@ -151,8 +170,41 @@ __device__ __forceinline__ void t_prob_blocks(const TrajCtx<T> c, int b0, int nc
 template <typename T>
 __device__ __forceinline__ double t_prob_final(const TrajCtx<T> c, int nb) {
     double *s = c.red;
-    double *bsum = c.red + 8 * kBlock;
+    double *bsum = block_sums(c);
     const int t = threadIdx.x;
+    if (nb > kTrajLdsSumBlocks) {
+        // k_reduce_f64 level by level: blocks of 4096 sums, lane t adding
+        // elements t + 256 j serially from 0.0, then the pairwise tree; the
+        // level's totals go after its inputs until one remains
+        double *in = c.gsum, *out = c.gsum + nb;
+        uint64_t m = (uint64_t)nb;
+        while (m > 1) {
+            const uint64_t nn = (m + kReduceSpan - 1) / kReduceSpan;
+            for (uint64_t b = 0; b < nn; ++b) {
+                double a = 0.0;
+#pragma unroll
+                for (int j = 0; j < kReduceSpan / kBlock; ++j) {
+                    const uint64_t e = b * kReduceSpan + (uint64_t)j * kBlock + t;
+                    if (e < m) a = a + in[e];
+                }
+                s[t] = a;
+                __syncthreads();
+                for (int st = kBlock / 2; st >= 1; st >>= 1) {
+                    if (t < st) s[t] = s[t] + s[t + st];
+                    __syncthreads();
+                }
+                if (t == 0) out[b] = s[0];
+                __syncthreads();
+            }
+            double *nx = in;
+            in = out;
+            out = nx;
+            m = nn;
+        }
+        const double r = in[0];
+        __syncthreads();
+        return r;
+    }
     if (nb > 1) {
         double a = 0.0;
         if (t < nb) a = a + bsum[t];  // the 4096-wide pass: only j = 0 is populated
@@ -314,6 +366,50 @@ __device__ __forceinline__ void t_kq(const TrajCtx<T> c, Pend<T> &p, const uint3
     __syncthreads();
 }
 
+// The same op for K = 4..6 without holding a group's 2^K amplitudes in
+// registers: lane t computes row t % D of group t / D of each batch of
+// 256 / D groups (a group's lanes share a wave, so its loads are
+// broadcasts), in kern_kq_scalar's column order, and every lane writes its
+// row only after the whole workgroup has read the batch.
+template <typename T, int K>
+__device__ __forceinline__ void t_kq_rows(const TrajCtx<T> c, Pend<T> &p, const uint32_t *qs, const T *m) {
+    constexpr int D = 1 << K, G = kBlock / D;
+    const int t = threadIdx.x, r = t % D;
+    const uint64_t groups = c.len >> K;
+    for (uint64_t g0 = 0; g0 < groups; g0 += G) {
+        const uint64_t g = g0 + (uint64_t)(t / D);
+        uint64_t base = 0, o = 0;
+        T ar = T(0), ai = T(0);
+        if (g < groups) {
+            base = g;
+#pragma unroll
+            for (int b = 0; b < K; ++b) base = insert_zero(base, qs[b]);
+            for (int cc = 0; cc < D; ++cc) {
+                uint64_t src = base;
+#pragma unroll
+                for (int b = 0; b < K; ++b)
+                    if ((cc >> b) & 1) src |= 1ull << qs[b];
+                T xr = c.re[src], xi = c.im[src];
+                pend_apply(p, src, xr, xi);
+                const T wr = m[r * D + cc], wi = m[D * D + r * D + cc];
+                ar = ar + wr * xr - wi * xi;
+                ai = ai + wr * xi + wi * xr;
+            }
+            o = base;
+#pragma unroll
+            for (int b = 0; b < K; ++b)
+                if ((r >> b) & 1) o |= 1ull << qs[b];
+        }
+        __syncthreads();
+        if (g < groups) {
+            c.re[o] = ar;
+            c.im[o] = ai;
+        }
+        __syncthreads();
+    }
+    p.n = 0;
+}
+
 // kern_diag1_scalar (src/state.rs:498-527): d0 on the a side, d1 on the b side.
 template <typename T>
 __device__ __forceinline__ void t_diag1(const TrajCtx<T> c, Pend<T> &p, uint32_t q, T d0r, T d1r, T d0i, T d1i) {
@@ -347,8 +443,8 @@ __device__ __forceinline__ void t_diag1(const TrajCtx<T> c, Pend<T> &p, uint32_t
     __syncthreads();
 }
 
-// Diagonal op; d = [re D | im D] (k == 1: d0r, d1r, d0i, d1i).
-template <typename T>
+// Diagonal op on k <= KQ qubits; d = [re D | im D] (k == 1: d0r, d1r, d0i, d1i).
+template <typename T, int KQ>
 __device__ __forceinline__ void t_diag(const TrajCtx<T> c, Pend<T> &p, const uint32_t *qs, int k, const T *d) {
     const int D = 1 << k;
     if (k == 1) {
@@ -370,7 +466,7 @@ __device__ __forceinline__ void t_diag(const TrajCtx<T> c, Pend<T> &p, const uin
                 if (i < c.len) {
                     uint32_t j = 0;
 #pragma unroll
-                    for (int b = 0; b < kTrajMaxK; ++b)
+                    for (int b = 0; b < KQ; ++b)
                         if (b < k) j |= (uint32_t)((i >> qs[b]) & 1) << b;
                     pend_apply(p, i, ar[u], ai[u]);
                     const T wr = d[j], wi = d[D + j];
@@ -505,21 +601,26 @@ __device__ __forceinline__ void t_pauli(const TrajCtx<T> c, Pend<T> &p, uint32_t
 }
 
 // p1_first: prob_one of qs[0] formed by the gate's own sweep (or < 0); it
-// stands until any Pauli changes the state.
-template <typename T>
+// stands until any Pauli changes the state.  k <= KQ.  Up to 3 qubits the
+// loops over the op's qubits unroll (qs in registers); above, they stay
+// loops over qs in the op record (uniform loads), one copy of their body.
+template <typename T, int KQ>
 __device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, Pend<T> &p, const TrajNoise &nz, const uint32_t *qs,
                                              int k, DevRng &rng, double p1_first = -1.0) {
     const double pd = k == 1 ? nz.dep1 : nz.dep2;
     if (pd > 0.0 && rng.f64() < pd) {
         const uint64_t idx = rng.range(1, 1ull << (2 * k));
+        if constexpr (KQ <= kTrajMaxK) {
 #pragma unroll
-        for (int b = 0; b < kTrajMaxK; ++b)
-            if (b < k) t_pauli(c, p, qs[b], (idx >> (2 * b)) & 3);
+            for (int b = 0; b < KQ; ++b)
+                if (b < k) t_pauli(c, p, qs[b], (idx >> (2 * b)) & 3);
+        } else {
+#pragma unroll 1
+            for (int b = 0; b < k; ++b) t_pauli(c, p, qs[b], (idx >> (2 * b)) & 3);
+        }
         p1_first = -1.0;
     }
-#pragma unroll
-    for (int b = 0; b < kTrajMaxK; ++b) {
-        if (b >= k) break;
+    auto channels = [&](const int b) __attribute__((always_inline)) {
         const uint32_t q = qs[b];
         if (nz.bit_flip > 0.0 && rng.f64() < nz.bit_flip) t_pauli(c, p, q, 1), p1_first = -1.0;
         if (nz.phase_flip > 0.0 && rng.f64() < nz.phase_flip) t_pauli(c, p, q, 3), p1_first = -1.0;
@@ -531,7 +632,7 @@ __device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, Pend<T> &p, con
                 t_pauli(c, p, q, 1);
             } else {  // diag(sc, sc*sqrt(1-gamma)) on q, held for the next sweep
                 const double sc = 1.0 / sqrt(1.0 - pj);
-                if (p.n == kTrajMaxK) t_pend_flush(c, p);  // not reached: a gate has k <= 3 qubits
+                if (p.n == kTrajMaxK) t_pend_flush(c, p);  // a gate on more than 3 qubits
                 const T d0 = (T)sc, d1 = (T)(sc * sqrt(1.0 - nz.damping));
 #pragma unroll
                 for (int e = 0; e < kTrajMaxK; ++e)  // static indices keep Pend in registers
@@ -539,6 +640,16 @@ __device__ __forceinline__ void t_gate_noise(const TrajCtx<T> c, Pend<T> &p, con
                 ++p.n;
             }
         }
+    };
+    if constexpr (KQ <= kTrajMaxK) {
+#pragma unroll
+        for (int b = 0; b < KQ; ++b) {
+            if (b >= k) break;
+            channels(b);
+        }
+    } else {
+#pragma unroll 1
+        for (int b = 0; b < k; ++b) channels(b);
     }
 }
 
@@ -555,19 +666,24 @@ __device__ __forceinline__ int t_readout(const TrajNoise &nz, int bit, DevRng &r
 // and that runs at the same grid.
 template <typename T>
 struct TrajLaunch {
-    T *states;           // gridDim.x states of 2 * 2^n values, packed
+    T *states;           // gridDim.x slots of `slot` values: re, im (2^n each), block sums
     const TrajOp *ops;
     const uint32_t *fm;  // final measures: (qubit, clbit) pairs
     const T *mats;
     uint64_t *out;       // per-shot clbit words
     uint64_t seed, shots;
+    uint64_t slot;       // values of T per workgroup slot
     TrajNoise nz;
     uint32_t n, nops, nfm;
     int32_t noisy;
 };
 
-template <typename T>
+template <typename T, int KM>
 __global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__restrict__ L) {
+    // ops on up to KQ qubits; the LDS stage holds a 2^KM dense matrix or a
+    // 2^KQ diagonal (the same 2 << 2KM values in the wide instance)
+    constexpr int KQ = KM == kTrajMaxK ? kTrajMaxK : kTrajOpQubits;
+    constexpr int kStage = (2 << (2 * KM)) > (2 << KQ) ? (2 << (2 * KM)) : (2 << KQ);
     T *__restrict__ ws = L->states;
     const uint32_t n = L->n, nops = L->nops, nfm = L->nfm;
     const TrajOp *__restrict__ ops = L->ops;
@@ -576,9 +692,9 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__
     uint64_t *__restrict__ out = L->out;
     const TrajNoise nz = L->nz;
     const int noisy = L->noisy;
-    const uint64_t seed = L->seed, shots = L->shots;
+    const uint64_t seed = L->seed, shots = L->shots, slot = L->slot;
     __shared__ double red[9 * kBlock];
-    __shared__ T smat[2 << (2 * kTrajMaxK)];
+    __shared__ T smat[kStage];
     __shared__ T spauli[16];
     if (threadIdx.x < 16) {
         // X = [[0,1],[1,0]], Y = [[0,-i],[i,0]], each as [re 4 | im 4]
@@ -588,8 +704,13 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__
     __syncthreads();
     const uint64_t len = 1ull << n;
     // a 1q gate's sweep also forms the damping probability of its qubit
-    const bool fuse_p1 = noisy && nz.damping > 0.0 && n >= 13 && n <= 20;
-    TrajCtx<T> c{ws + (uint64_t)blockIdx.x * 2 * len, ws + (uint64_t)blockIdx.x * 2 * len + len, len, red, spauli};
+    const bool fuse_p1 = noisy && nz.damping > 0.0 && n >= 13;
+    // this workgroup's slot: re, im, then (above 21 qubits) the block sums
+    T *const mine = ws + (uint64_t)blockIdx.x * slot;
+    double *const gsum = (len >> 1) / kReduceSpan > (uint64_t)kTrajLdsSumBlocks
+                             ? reinterpret_cast<double *>(mine + 2 * len)
+                             : nullptr;
+    TrajCtx<T> c{mine, mine + len, len, red, spauli, gsum};
     for (uint64_t shot = blockIdx.x; shot < shots; shot += gridDim.x) {
         for (uint64_t i = threadIdx.x; i < len; i += blockDim.x) {
             c.re[i] = i == 0 ? T(1) : T(0);
@@ -607,7 +728,12 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__
                 if (((clbits >> op.creg_off) & mask) != op.value) continue;
             }
             const uint32_t kind = op.kind, k = op.k;
-            const uint32_t qs[kTrajMaxK] = {op.q[0], op.q[1], op.q[2]};
+            // the narrow instance holds the op's qubits in registers; the
+            // wide one reads them from the op record
+            uint32_t qreg[kTrajMaxK];
+#pragma unroll
+            for (int b = 0; b < kTrajMaxK; ++b) qreg[b] = op.q[b];
+            const uint32_t *qs = KM == kTrajMaxK ? qreg : op.q;
             if (kind <= 1) {  // stage the matrix / diagonal in LDS
                 const uint32_t cnt = kind == 0 ? 2u << (2 * k) : 2u << k;
                 for (uint32_t i = threadIdx.x; i < cnt; i += blockDim.x) smat[i] = mats[op.data_off + i];
@@ -618,19 +744,24 @@ __global__ __launch_bounds__(kBlock) void k_trajectories(const TrajLaunch<T> *__
                     double p1 = -1.0;
                     if (k == 1) p1 = t_1q(c, pend, qs[0], smat, fuse_p1);
                     else if (k == 2) t_kq<T, 2>(c, pend, qs, smat);
-                    else t_kq<T, 3>(c, pend, qs, smat);
-                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng, p1);
+                    else if (k == 3) t_kq<T, 3>(c, pend, qs, smat);
+                    else if constexpr (KM > kTrajMaxK) {
+                        if (k == 4) t_kq_rows<T, 4>(c, pend, qs, smat);
+                        else if (k == 5) t_kq_rows<T, 5>(c, pend, qs, smat);
+                        else t_kq_rows<T, 6>(c, pend, qs, smat);
+                    }
+                    if (noisy) t_gate_noise<T, KQ>(c, pend, nz, qs, (int)k, rng, p1);
                     break;
                 }
                 case 1:
-                    t_diag(c, pend, qs, (int)k, smat);
-                    if (noisy) t_gate_noise(c, pend, nz, qs, (int)k, rng);
+                    t_diag<T, KQ>(c, pend, qs, (int)k, smat);
+                    if (noisy) t_gate_noise<T, KQ>(c, pend, nz, qs, (int)k, rng);
                     break;
                 case 2: {
                     t_cx(c, pend, qs[0], qs[1]);
                     if (noisy) {
-                        const uint32_t sq[kTrajMaxK] = {qs[0] < qs[1] ? qs[0] : qs[1], qs[0] < qs[1] ? qs[1] : qs[0], 0};
-                        t_gate_noise(c, pend, nz, sq, 2, rng);
+                        const uint32_t sq[2] = {qs[0] < qs[1] ? qs[0] : qs[1], qs[0] < qs[1] ? qs[1] : qs[0]};
+                        t_gate_noise<T, 2>(c, pend, nz, sq, 2, rng);
                     }
                     break;
                 }

@@ -236,7 +236,7 @@ class RunOptions:
     mem_fraction: float = 0.75  # src/exec.rs:65
     want_statevector: bool = False
     noise: Optional[NoiseModel] = None
-    mode: int = HipBackend.TILED
+    mode: int = HipBackend.TILED  # 0 exact, 1 tiled, 2 tiled with the per-shot loop on the host (zr_run)
 
 
 def resolve_precision(n_qubits: int, opts: RunOptions):
@@ -272,7 +272,7 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
         import time
 
         t0 = time.perf_counter()
-        with HipBackend(c.n_qubits, prec, opts.mode) as be:
+        with HipBackend(c.n_qubits, prec, HipBackend.TILED if opts.mode == 2 else opts.mode) as be:
             be.apply_compiled(c)
             be.finish()
             statevector = be.statevector()
@@ -315,8 +315,8 @@ def run_compiled(c: Compiled, opts: RunOptions) -> RunResult:
 
 
 def trajectories_supported(c: Compiled) -> bool:
-    """Whether `run_trajectories` takes the circuit: <= 20 qubits and
-    unitary/diagonal ops on <= 3 qubits."""
+    """Whether `run_trajectories` takes the circuit: <= 26 qubits, dense ops
+    on <= 6 qubits, diagonals on <= 12."""
     return bool(lib().zr_trajectories_supported(c.handle))
 
 
