@@ -484,6 +484,26 @@ struct WinHeader {
 
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
 
+// The window's ops run in a loop over a runtime op list, so the lane's
+// registers are loop-carried and every op case must leave each value in the
+// register it came in: a case whose results land elsewhere makes the
+// register allocator put a full set of copies on the loop's common latch,
+// paid by every op.  Hence (1) the arithmetic ops finish every input read
+// before the first result is written (a result can then take its input's
+// register), and (2) pure permutations swap in place through tied asm
+// operands.
+
+// Swap two register values in place (v_swap_b32 per 32-bit half).
+template <typename T>
+__device__ __forceinline__ void rswap(T &a, T &b) {
+    if constexpr (sizeof(T) == 8) {
+        uint32_t *pa = reinterpret_cast<uint32_t *>(&a), *pb = reinterpret_cast<uint32_t *>(&b);
+        asm volatile("v_swap_b32 %0, %1\n\tv_swap_b32 %2, %3" : "+v"(pa[0]), "+v"(pb[0]), "+v"(pa[1]), "+v"(pb[1]));
+    } else {
+        asm volatile("v_swap_b32 %0, %1" : "+v"(a), "+v"(b));
+    }
+}
+
 template <typename T, int NS, int S, bool F>
 __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
     const T m00r = d[0], m01r = d[1], m10r = d[2], m11r = d[3];
@@ -493,10 +513,24 @@ __device__ __forceinline__ void reg_1q(T *xr, T *xi, const T *__restrict__ d) {
         if ((r >> S) & 1) continue;
         const int a = r, c = r | (1 << S);
         const T ar = xr[a], ai = xi[a], br = xr[c], bi = xi[c];
-        xr[a] = msc<F>(mac<F>(msc<F>(m00r * ar, m00i, ai), m01r, br), m01i, bi);
-        xi[a] = mac<F>(mac<F>(mac<F>(m00r * ai, m00i, ar), m01r, bi), m01i, br);
-        xr[c] = msc<F>(mac<F>(msc<F>(m10r * ar, m10i, ai), m11r, br), m11i, bi);
-        xi[c] = mac<F>(mac<F>(mac<F>(m10r * ai, m10i, ar), m11r, bi), m11i, br);
+        if constexpr (F) {
+            xr[a] = msc<F>(mac<F>(msc<F>(m00r * ar, m00i, ai), m01r, br), m01i, bi);
+            xi[a] = mac<F>(mac<F>(mac<F>(m00r * ai, m00i, ar), m01r, bi), m01i, br);
+            xr[c] = msc<F>(mac<F>(msc<F>(m10r * ar, m10i, ai), m11r, br), m11i, bi);
+            xi[c] = mac<F>(mac<F>(mac<F>(m10r * ai, m10i, ar), m11r, bi), m11i, br);
+        } else {
+            // the same roundings as ((m00r*ar - m00i*ai) + m01r*br) - m01i*bi,
+            // with each last product formed before any result is written
+            const T p0 = mac<F>(msc<F>(m00r * ar, m00i, ai), m01r, br);
+            const T p1 = mac<F>(mac<F>(m00r * ai, m00i, ar), m01r, bi);
+            const T p2 = mac<F>(msc<F>(m10r * ar, m10i, ai), m11r, br);
+            const T p3 = mac<F>(mac<F>(m10r * ai, m10i, ar), m11r, bi);
+            const T q0 = m01i * bi, q1 = m01i * br, q2 = m11i * bi, q3 = m11i * br;
+            xr[a] = p0 - q0;
+            xi[a] = p1 + q1;
+            xr[c] = p2 - q2;
+            xi[c] = p3 + q3;
+        }
     }
 }
 
@@ -555,11 +589,8 @@ __device__ __forceinline__ void reg_cx_slot(T *xr, T *xi) {
     for (int r = 0; r < (1 << NS); ++r) {
         if (((r >> ST) & 1) || !((r >> SC) & 1)) continue;
         const int c = r | (1 << ST);
-        const T t0 = xr[r], t1 = xi[r];
-        xr[r] = xr[c];
-        xi[r] = xi[c];
-        xr[c] = t0;
-        xi[c] = t1;
+        rswap(xr[r], xr[c]);
+        rswap(xi[r], xi[c]);
     }
 }
 
@@ -570,25 +601,46 @@ __device__ __forceinline__ void reg_cx_swap(T *xr, T *xi) {
     for (int r = 0; r < (1 << NS); ++r) {
         if ((r >> ST) & 1) continue;
         const int c = r | (1 << ST);
-        const T a0 = xr[r], a1 = xi[r];
-        xr[r] = xr[c];
-        xi[r] = xi[c];
-        xr[c] = a0;
-        xi[c] = a1;
+        rswap(xr[r], xr[c]);
+        rswap(xi[r], xi[c]);
     }
 }
 
+// Swap the re and im values of two registers in the lanes of mask m only:
+// exec is narrowed to m around plain in-place swaps and restored in the same
+// asm block (every lane of a window workgroup is active here).
+template <typename T>
+__device__ __forceinline__ void mswap(T &ar, T &ai, T &br, T &bi, uint64_t m) {
+    uint64_t sv;
+    if constexpr (sizeof(T) == 8) {
+        uint32_t *a0 = reinterpret_cast<uint32_t *>(&ar), *a1 = reinterpret_cast<uint32_t *>(&ai);
+        uint32_t *b0 = reinterpret_cast<uint32_t *>(&br), *b1 = reinterpret_cast<uint32_t *>(&bi);
+        asm volatile(
+            "s_and_saveexec_b64 %[sv], %[m]\n\t"
+            "v_swap_b32 %0, %1\n\tv_swap_b32 %2, %3\n\tv_swap_b32 %4, %5\n\tv_swap_b32 %6, %7\n\t"
+            "s_mov_b64 exec, %[sv]"
+            : "+v"(a0[0]), "+v"(b0[0]), "+v"(a0[1]), "+v"(b0[1]), "+v"(a1[0]), "+v"(b1[0]), "+v"(a1[1]),
+              "+v"(b1[1]), [sv] "=&s"(sv)
+            : [m] "s"(m)
+            : "scc");
+    } else {
+        asm volatile(
+            "s_and_saveexec_b64 %[sv], %[m]\n\tv_swap_b32 %0, %1\n\tv_swap_b32 %2, %3\n\ts_mov_b64 exec, %[sv]"
+            : "+v"(ar), "+v"(br), "+v"(ai), "+v"(bi), [sv] "=&s"(sv)
+            : [m] "s"(m)
+            : "scc");
+    }
+}
+
+// CX whose control is a lane bit: swap along the target slot in the lanes
+// whose control bit is set (m: their lane mask).
 template <typename T, int NS, int ST>
-__device__ __forceinline__ void reg_cx_all(T *xr, T *xi, bool on) {
+__device__ __forceinline__ void reg_cx_all(T *xr, T *xi, uint64_t m) {
 #pragma unroll
     for (int r = 0; r < (1 << NS); ++r) {
         if ((r >> ST) & 1) continue;
         const int c = r | (1 << ST);
-        const T a0 = xr[r], a1 = xi[r], b0 = xr[c], b1 = xi[c];
-        xr[r] = on ? b0 : a0;
-        xi[r] = on ? b1 : a1;
-        xr[c] = on ? a0 : b0;
-        xi[c] = on ? a1 : b1;
+        mswap(xr[r], xi[r], xr[c], xi[c], m);
     }
 }
 
@@ -789,7 +841,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
                 // register state twice and the merges cost register copies
                 constexpr uint32_t k1 = kW1q << 8, kc = kWCx << 8, ks = kCtrlSlot << 4, kl = kCtrlLocal << 4,
                                    kf = kCtrlFixed << 4;
-                const bool on = (lb >> op->aux) & 1;               // kCtrlLocal
+                const uint64_t on = __builtin_amdgcn_ballot_w64((lb >> op->aux) & 1);  // kCtrlLocal lanes
                 const bool fixed_on = (base >> op->aux) & 1;       // kCtrlFixed: uniform
                 switch (kind << 8 | sel) {
                     case k1 | 0: reg_1q<T, NS, 0, F>(xr, xi, mats + op->data_off); break;
@@ -808,8 +860,8 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
                     case kc | ks | 3 | 0 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 0>(xr, xi); break;
                     case kc | ks | 3 | 1 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 1>(xr, xi); break;
                     case kc | ks | 3 | 2 << 2: if constexpr (NS > 3) reg_cx_slot<T, NS, 3, 2>(xr, xi); break;
-                    // control on a lane bit: per-lane selects; fixed for the
-                    // tile: a uniform branch around plain swaps
+                    // control on a lane bit: swaps in the lanes whose bit is set;
+                    // fixed for the tile: a uniform branch around plain swaps
                     case kc | kl | 0: reg_cx_all<T, NS, 0>(xr, xi, on); break;
                     case kc | kl | 1: reg_cx_all<T, NS, 1>(xr, xi, on); break;
                     case kc | kl | 2: if constexpr (NS > 2) reg_cx_all<T, NS, 2>(xr, xi, on); break;
@@ -848,8 +900,14 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
                                 for (int i = 0; i < NS; ++i) j |= ((r >> i) & 1) ? js[i] : 0u;
                                 const T wr = d[j], wi = d[D + j];
                                 const T ar = xr[r], ai = xi[r];
-                                xr[r] = msc<F>(wr * ar, wi, ai);
-                                xi[r] = mac<F>(wr * ai, wi, ar);
+                                if constexpr (F) {
+                                    xr[r] = msc<F>(wr * ar, wi, ai);
+                                    xi[r] = mac<F>(wr * ai, wi, ar);
+                                } else {  // same roundings, inputs read first
+                                    const T p0 = wr * ar, q0 = wi * ai, p1 = wr * ai, q1 = wi * ar;
+                                    xr[r] = p0 - q0;
+                                    xi[r] = p1 + q1;
+                                }
                             }
                         }
                         break;
