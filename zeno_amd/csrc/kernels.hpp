@@ -482,6 +482,17 @@ struct WinHeader {
     uint32_t wb[kMaxTileBits];
 };
 
+template <typename T>
+struct Cplx;
+template <>
+struct Cplx<double> {
+    using type = double2;
+};
+template <>
+struct Cplx<float> {
+    using type = float2;
+};
+
 __device__ __forceinline__ int padded(int l) { return l + (l >> 4); }
 
 // The window's ops run in a loop over a runtime op list, so the lane's
@@ -703,7 +714,25 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
     using Shape = WinShape<T, TB, NS>;
     constexpr int B = Shape::kLanes, R = Shape::kRegs;
     constexpr int LB = TB - NS;  // local bits carried by the lane index
-    __shared__ T sre[Shape::kTilePad], sim[Shape::kTilePad];
+    // f64: re and im of an amplitude side by side, one 16-byte LDS access
+    // per amplitude (3% faster per window than two 8-byte arrays); f32: two
+    // arrays (the interleaved 8-byte form measured 3% slower)
+    using T2 = typename Cplx<T>::type;
+    constexpr bool kInter = sizeof(T) == 8;
+    __shared__ T2 sx[Shape::kTilePad];
+    T *const sre = reinterpret_cast<T *>(sx), *const sim = sre + Shape::kTilePad;
+    auto lds_put = [&](int i, T a, T b) {
+        if constexpr (kInter) sx[i] = T2{a, b};
+        else sre[i] = a, sim[i] = b;
+    };
+    auto lds_get = [&](int i, T &a, T &b) {
+        if constexpr (kInter) {
+            const T2 v = sx[i];
+            a = v.x, b = v.y;
+        } else {
+            a = sre[i], b = sim[i];
+        }
+    };
     const int t = threadIdx.x;
     const WinHeader *hdr = reinterpret_cast<const WinHeader *>(blob);
     const WinStage *stages = reinterpret_cast<const WinStage *>(blob + hdr->stage_off);
@@ -789,8 +818,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
         }
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            sre[lt + (B + B / 16) * j] = pr[j];
-            sim[lt + (B + B / 16) * j] = pi[j];
+            lds_put(lt + (B + B / 16) * j, pr[j], pi[j]);
         }
         if (pending) {
 #pragma unroll
@@ -830,8 +858,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
                 int c = 0;
 #pragma unroll
                 for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
-                xr[r] = sre[plb + padded(c)];
-                xi[r] = sim[plb + padded(c)];
+                lds_get(plb + padded(c), xr[r], xi[r]);
             }
             const int oe = (int)st->op_end;
             for (int o = (int)st->op_begin; o < oe; ++o) {
@@ -918,8 +945,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
                 int c = 0;
 #pragma unroll
                 for (int i = 0; i < NS; ++i) c |= ((r >> i) & 1) << sl[i];
-                sre[plb + padded(c)] = xr[r];
-                sim[plb + padded(c)] = xi[r];
+                lds_put(plb + padded(c), xr[r], xi[r]);
             }
             __syncthreads();
         }
@@ -930,8 +956,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
         asm volatile("" : "+v"(w));
 #pragma unroll
         for (int j = 0; j < R; ++j) {
-            wr[j] = sre[w + woff[j]];
-            wi[j] = sim[w + woff[j]];
+            lds_get(w + woff[j], wr[j], wi[j]);
         }
         wbase = base;
         pending = true;
