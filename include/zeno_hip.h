@@ -205,9 +205,12 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * ops arrive through zr_apply_ops or a run, a window that closes because
  * the next op does not fit also moves onto the low state bits, on
  * write-back, the qubits of its tile that the coming ops need soonest; a
- * relabelling permutes state index bits only) and
+ * relabelling permutes state index bits only),
  * "layout" (-1 by size, 0 identity, 1 spread, >= 2 a seeded shuffle;
- * resets the state to |0...0>).  None of them changes results, except
+ * resets the state to |0...0>) and "chunk_multi" (>= 0, default 4096:
+ * from this many 2^16-value chunks the sampler's pass 1 walks 8 chunks per
+ * workgroup instead of one; 0 = never; the sums are the same serial
+ * chains either way).  None of them changes results, except
  * "mfma" (0/1, default 0): dense ops on 4 to 6 qubits run as real GEMMs on
  * the matrix cores (v_mfma_*_16x16x4) when n >= 12.  MFMA accumulation
  * rounds differently from the CPU path's sum, so those ops agree with it

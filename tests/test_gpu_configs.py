@@ -87,6 +87,20 @@ def test_precision_config_30q_depth40():
     ref.close()
 
 
+@pytest.mark.parametrize("n,prec", [(30, 64), (29, 32), (28, 64)])
+def test_sampler_pass1_walks_8_chunks_per_workgroup_bit_exact(n, prec):
+    """From 4096 chunks sampler pass 1 walks 8 chunks per workgroup; its
+    chunk sums and a 1000-shot sample equal the one-chunk-per-workgroup
+    walk's (chunk_multi=0), which equals the oracle's (27-qubit test)."""
+    prog = z.matching_circuit(n, 2, 11 + n, rzz_period=0).to_program()[0]
+    with _run(n, prec, z.compile(prog, n, fusion=1)) as g:
+        _say(f"sampler pass 1 n={n} f{prec}")
+        multi, draws = g.chunk_sums().view(np.uint64), g.sample(1000, 21)
+        g.tune("chunk_multi", 0)
+        assert np.array_equal(multi, g.chunk_sums().view(np.uint64))
+        assert np.array_equal(draws, g.sample(1000, 21))
+
+
 def test_precision_config_33q_depth40():
     """33 qubits, depth 40 (128 GiB f64 and 64 GiB f32 resident together):
     f32 bulk within 1e-5 of exact f64, f64 norm within 1e-12."""

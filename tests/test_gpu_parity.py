@@ -339,7 +339,9 @@ def test_sampler_many_chunks_bit_exact(orc):
     """At 27 qubits (2048 chunks, 8 per CU) the chunk walks take their
     one-wave shape, and so does the rescan of a 100000-shot sample (device
     sort, a block per chunk); 1000 shots rescan the touched chunks with the
-    four-wave shape.  Chunk sums and draws equal the oracle's."""
+    four-wave shape.  With chunk_multi=1024 pass 1 walks 8 chunks per
+    workgroup instead (the default from 4096 chunks).  Chunk sums and draws
+    equal the oracle's."""
     n = 27
     prog = orc.gen_random(n, 2, 2700)
     orc.set_threads(16)
@@ -348,9 +350,12 @@ def test_sampler_many_chunks_bit_exact(orc):
             _say(f"sampler many chunks n={n} prec={prec}")
             o = orc.State(n, prec)
             o.apply_compiled(orc.Compiled(prog, n, fusion=2))
+            want = o.chunk_sums().view(np.uint64)
             with z.HipBackend(n, prec) as g:
                 g.apply_compiled(z.compile(prog, n, fusion=2))
-                assert np.array_equal(g.chunk_sums().view(np.uint64), o.chunk_sums().view(np.uint64))
+                for multi in (0, 1024):
+                    g.tune("chunk_multi", multi)
+                    assert np.array_equal(g.chunk_sums().view(np.uint64), want), multi
                 for shots, seed in ((100000, 5), (1000, 6)):
                     assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed)), (prec, shots)
             del o

@@ -1178,6 +1178,74 @@ __global__ __launch_bounds__(S::kBlock) void k_chunk_sums(const T *__restrict__ 
     if (threadIdx.x == 0) sums[blockIdx.x] = acc;
 }
 
+// sample_indices pass 1 for many chunks (src/sample.rs:38-50): a workgroup
+// owns G consecutive full chunks of 2^16 values and lane g (g < G) walks
+// chunk g, so G serial chains share one wave instead of one chain per
+// workgroup.  Each slab holds S = SLAB / G values of every chunk: the 256
+// lanes load it coalesced along each chunk (registers, issued while wave 0
+// walks the previous slab), square it and store it transposed in LDS (value
+// i of chunk g at i * (G + 1) + g: the chain reads are consecutive across
+// lanes, the stores stride G + 1 doubles), double-buffered.  Every chain
+// adds r*r + i*i (in f64) in index order, exactly the reference's order.
+template <typename T, int G, int SLAB>
+__global__ __launch_bounds__(256) void k_chunk_sums_multi(const T *__restrict__ re, const T *__restrict__ im,
+                                                          Layout L, uint64_t nch, double *__restrict__ sums) {
+    constexpr int kLanes = 256, kChunk = 1 << 16, S = SLAB / G, kPer = S * G / kLanes, kRow = G + 1;
+    static_assert(kPer * kLanes == S * G && kChunk % S == 0, "whole slabs");
+    __shared__ double tv[2][S * kRow];
+    __shared__ uint64_t hp[G];  // physical bits of each chunk's base (bits 16 and up)
+    const uint64_t c0 = (uint64_t)blockIdx.x * G;
+    const int gc = nch - c0 < (uint64_t)G ? (int)(nch - c0) : G;
+    const int t = threadIdx.x;
+    if (t < G) {
+        const uint64_t b = (c0 + t) << 16;
+        hp[t] = L.lut ? L.lut[512 + ((b >> 16) & 255)] | L.lut[768 + ((b >> 24) & 255)] |
+                            L.lut[1024 + ((b >> 32) & 255)]
+                      : b;
+    }
+    __syncthreads();
+    // value u of the lane: chunk g = e / S, offset i = e % S within the slab
+    T r[kPer], m[kPer];
+    auto load = [&](int off) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kLanes, g = e / S, x = off + e % S;
+            if (g < gc) {
+                const uint64_t lo = L.lut ? L.lut[x & 255] | L.lut[256 + (x >> 8)] : (uint64_t)x;
+                const uint64_t p = hp[g] | lo;
+                r[u] = re[p];
+                m[u] = im[p];
+            }
+        }
+    };
+    auto store = [&](double *buf) {
+#pragma unroll
+        for (int u = 0; u < kPer; ++u) {
+            const int e = t + u * kLanes, g = e / S, i = e % S;
+            if (g < gc) {
+                const double a = (double)r[u], b = (double)m[u];
+                buf[i * kRow + g] = a * a + b * b;
+            }
+        }
+    };
+    double acc = 0.0;
+    load(0);
+    store(tv[0]);
+    __syncthreads();
+    for (int off = 0, k = 0; off < kChunk; off += S, k ^= 1) {
+        const bool more = off + S < kChunk;
+        if (more) load(off + S);
+        if (t < gc) {
+            const double *col = tv[k] + t;
+#pragma unroll 16
+            for (int i = 0; i < S; ++i) acc += col[i * kRow];
+        }
+        if (more) store(tv[k ^ 1]);
+        __syncthreads();
+    }
+    if (t < gc) sums[c0 + t] = acc;
+}
+
 // sample_indices pass 5 + 6 (src/sample.rs:68-107): rescan one touched
 // chunk from cum_before[c], emitting the sorted draws us[s0..s1) that fall
 // below the running sum; leftovers land on the chunk's last index with
