@@ -74,6 +74,7 @@ def lib():
             "zo_rng_f64_stream": (None, [U64, U64, P]),
             "zo_xoshiro_from_state": (None, [P, U64, P]),
             "zo_rng_range_stream": (None, [U64, U64, U64, U64, P]),
+            "zo_rng_range_from_state": (None, [P, U64, U64, U64, P]),
             "zo_gate_build": (I32, [C.c_char_p, P, I32, P, P]),
             "zo_gate_lookup": (I32, [C.c_char_p, P, P]),
             "zo_gen_random": (I64, [U32, U32, U64, P, I64]),
@@ -136,6 +137,13 @@ def xoshiro_from_state(state, n: int) -> np.ndarray:
     st = np.asarray(state, np.uint64)
     out = np.empty(n, np.uint64)
     lib().zo_xoshiro_from_state(_ptr(st), n, _ptr(out))
+    return out
+
+
+def rng_range_from_state(state, lo: int, hi: int, n: int) -> np.ndarray:
+    st = np.asarray(state, np.uint64)
+    out = np.empty(n, np.uint64)
+    lib().zo_rng_range_from_state(_ptr(st), lo, hi, n, _ptr(out))
     return out
 
 

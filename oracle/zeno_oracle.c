@@ -7,8 +7,11 @@
  * Every function cites the reference file:line it restates.  Pins:
  *   rng        — rand_xoshiro's published xoshiro256++ test vector and the
  *                SplitMix64 reference outputs (tests/test_oracle.py);
- *                gen_range: parity unpinned (rand 0.8 is not in the tree;
- *                only statistical checks hold it).
+ *                gen_range (rand 0.8.7 UniformInt<u64>, not in the tree):
+ *                restated from its published algorithm and held by
+ *                crafted-state vectors on the rejection zone's edges plus a
+ *                uniformity check; parity against the crate's own outputs
+ *                is unpinned.
  *   gates      — hand-coded standard matrices (tests/reference.rs:157-283
  *                restated independently in numpy), unitarity.
  *   compiler   — compiler.rs:645-726 KATs; RefSim at every fusion level.
@@ -108,7 +111,7 @@ static double rng_f64(rng_t *r) {
 
 /* rand 0.8 UniformInt<u64>::sample_single(lo, hi) = sample_single_inclusive
  * (lo, hi-1): widening multiply with the "leading zeros" rejection zone.
- * Used by noise.rs:158.  Parity unpinned (see file header). */
+ * Used by noise.rs:158 (see the file header for what pins it). */
 static uint64_t rng_range(rng_t *r, uint64_t lo, uint64_t hi) {
     uint64_t range = hi - lo;
     if (range == 0) return rng_next(r);
@@ -138,6 +141,14 @@ void zo_rng_f64_stream(uint64_t seed, uint64_t n, double *out) {
     rng_t r;
     rng_seed(&r, seed);
     for (uint64_t i = 0; i < n; i++) out[i] = rng_f64(&r);
+}
+/* rng_range draws from an explicit generator state (the boundary tests
+ * craft states whose first output lands on the rejection zone's edge). */
+void zo_rng_range_from_state(const uint64_t *st, uint64_t lo, uint64_t hi, uint64_t n,
+                             uint64_t *out) {
+    rng_t r;
+    for (int i = 0; i < 4; i++) r.s[i] = st[i];
+    for (uint64_t i = 0; i < n; i++) out[i] = rng_range(&r, lo, hi);
 }
 void zo_rng_range_stream(uint64_t seed, uint64_t lo, uint64_t hi, uint64_t n,
                          uint64_t *out) {

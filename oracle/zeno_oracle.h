@@ -60,6 +60,8 @@ void zo_rng_f64_stream(uint64_t seed, uint64_t n, double *out);
 void zo_xoshiro_from_state(const uint64_t *st, uint64_t n, uint64_t *out);
 void zo_rng_range_stream(uint64_t seed, uint64_t lo, uint64_t hi, uint64_t n,
                          uint64_t *out);
+void zo_rng_range_from_state(const uint64_t *st, uint64_t lo, uint64_t hi, uint64_t n,
+                             uint64_t *out);
 
 /* ---- gates -------------------------------------------------------- */
 /* Writes the matrix (interleaved re,im; 2*4^k doubles for a unitary,

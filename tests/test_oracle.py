@@ -53,6 +53,56 @@ def test_gen_f64_range_and_moments(orc):
     assert np.all(u[:4] == (raw >> np.uint64(11)).astype(np.float64) / 2.0**53)
 
 
+def _state_with_first_output(v, s0=0x243F6A8885A308D3, s1=0x13198A2E03707344, s2=0xA4093822299F31D0):
+    """A xoshiro256++ state whose first output is v: the output is
+    rotl(s0 + s3, 23) + s0, so s3 = rotr(v - s0, 23) - s0."""
+    M = (1 << 64) - 1
+    x = (v - s0) & M
+    s3 = (((x >> 23) | (x << 41)) & M) - s0
+    return [s0, s1, s2, s3 & M]
+
+
+def test_gen_range_on_the_rejection_zone_edges(orc):
+    """rand 0.8 UniformInt<u64>::sample_single(lo, hi) (uniform.rs,
+    sample_single_inclusive(lo, hi - 1)): the widening product v * range
+    splits into (hi64, lo64); lo64 <= zone accepts and returns lo + hi64,
+    where zone = (range << leading_zeros(range)) - 1; otherwise the next
+    output is drawn.  States crafted so the first output sits exactly on
+    the zone's last accepted value and on the first rejected one catch an
+    off-by-one in the zone, a strict comparison, the low word returned in
+    place of the high one, or a missing redraw."""
+    M = (1 << 64) - 1
+    for lo, hi in ((1, 4), (1, 16), (1, 64), (5, 8)):
+        r = hi - lo
+        zone = ((r << (64 - r.bit_length())) & M) - 1
+        inv = pow(r, -1, 1 << 64)
+        for lo64, accepted in ((zone, True), (zone + 1, False), (0, True), (M, False)):
+            v = (lo64 * inv) & M
+            assert (v * r) & M == lo64
+            st = _state_with_first_output(v)
+            raw = [int(x) for x in orc.xoshiro_from_state(st, 8)]
+            assert raw[0] == v
+            got = int(orc.rng_range_from_state(st, lo, hi, 1)[0])
+            if accepted:
+                assert got == lo + ((v * r) >> 64), (lo, hi, lo64)
+            else:
+                nxt = next(w for w in raw[1:] if (w * r) & M <= zone)
+                assert got == lo + ((nxt * r) >> 64), (lo, hi, lo64)
+
+
+def test_gen_range_zone_rule_is_unbiased_by_brute_force():
+    """The zone rule (accept lo <= (range << lz(range)) - 1) on 8-bit words,
+    every range and every word: each output value is reached by the same
+    number of accepted words, the property that fixes the rule."""
+    for r in range(1, 256):
+        zone = ((r << (8 - r.bit_length())) & 255) - 1
+        counts = [0] * r
+        for v in range(256):
+            if (v * r) & 255 <= zone:
+                counts[(v * r) >> 8] += 1
+        assert min(counts) == max(counts) > 0, r
+
+
 def test_gen_range_uniform(orc):
     for hi in (4, 16, 64):
         v = orc.rng_range(99, 1, hi, 60_000)
