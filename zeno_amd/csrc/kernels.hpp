@@ -438,7 +438,7 @@ static __global__ __launch_bounds__(kBlock) void k_reduce_f64(const double *__re
 // amplitudes sharing every index bit outside the window's qubit set Q
 // (|Q| = 12, Q sorted; local bit i <-> qubit Q[i]; Q holds the lowest
 // qubits, so a tile's HBM accesses are 128-byte runs).  The tile is staged
-// in LDS with one pad element per 16 (bank spread).  The window is cut into
+// in LDS as (re, im) pairs with one pad pair per 16 (bank spread).  The window is cut into
 // stages: a stage fixes NS local "slot" bits; each lane gathers the 2^NS
 // amplitudes that differ only in those bits into registers, applies the
 // stage's ops there with exactly the per-op kernels' arithmetic, and
@@ -714,24 +714,15 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
     using Shape = WinShape<T, TB, NS>;
     constexpr int B = Shape::kLanes, R = Shape::kRegs;
     constexpr int LB = TB - NS;  // local bits carried by the lane index
-    // f64: re and im of an amplitude side by side, one 16-byte LDS access
-    // per amplitude (3% faster per window than two 8-byte arrays); f32: two
-    // arrays (the interleaved 8-byte form measured 3% slower)
+    // re and im of an amplitude side by side: one 16-byte (f64) or 8-byte
+    // (f32) LDS access per amplitude (same-box A/B against two arrays on
+    // the 30q circuit: f64 windows 3% and f32 windows 1.5% faster)
     using T2 = typename Cplx<T>::type;
-    constexpr bool kInter = sizeof(T) == 8;
     __shared__ T2 sx[Shape::kTilePad];
-    T *const sre = reinterpret_cast<T *>(sx), *const sim = sre + Shape::kTilePad;
-    auto lds_put = [&](int i, T a, T b) {
-        if constexpr (kInter) sx[i] = T2{a, b};
-        else sre[i] = a, sim[i] = b;
-    };
+    auto lds_put = [&](int i, T a, T b) { sx[i] = T2{a, b}; };
     auto lds_get = [&](int i, T &a, T &b) {
-        if constexpr (kInter) {
-            const T2 v = sx[i];
-            a = v.x, b = v.y;
-        } else {
-            a = sre[i], b = sim[i];
-        }
+        const T2 v = sx[i];
+        a = v.x, b = v.y;
     };
     const int t = threadIdx.x;
     const WinHeader *hdr = reinterpret_cast<const WinHeader *>(blob);
