@@ -101,6 +101,28 @@ def test_sampler_pass1_walks_8_chunks_per_workgroup_bit_exact(n, prec):
         assert np.array_equal(draws, g.sample(1000, 21))
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_fused_blocks_in_windows_on_the_spread_layout_bit_exact(prec):
+    """26-qubit brickwork at fusion 2, 3 and 4 on the default (spread)
+    layout, where a block's qubits land on state bits out of order: the
+    window stages order their register slots by each block's matrix bits
+    (4-qubit blocks with 4 slots), so every block runs inside a window, and
+    the state equals the per-op exact kernels' bit for bit."""
+    n = 26
+    prog = z.random_circuit(n, 8, 26).to_program()[0]
+    for fusion in (2, 3, 4):
+        comp = z.compile(prog, n, fusion=fusion)
+        _say(f"fused blocks n={n} f{prec} fusion={fusion}")
+        with z.HipBackend(n, prec, z.HipBackend.TILED) as g, z.HipBackend(n, prec, z.HipBackend.EXACT) as e:
+            assert g.layout() != list(range(n))  # the spread layout
+            g.profile_begin()
+            g.apply_compiled(comp)
+            kinds = {name for name, _, _ in g.profile_end()}
+            assert not kinds & {"k_kq<3>", "k_kq<4>"}, kinds  # 3- and 4-qubit blocks run in windows
+            e.apply_compiled(comp)
+            assert g.max_abs_diff(e) == 0.0
+
+
 def test_precision_config_33q_depth40():
     """33 qubits, depth 40 (128 GiB f64 and 64 GiB f32 resident together):
     f32 bulk within 1e-5 of exact f64, f64 norm within 1e-12."""

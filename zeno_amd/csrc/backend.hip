@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -550,14 +551,6 @@ void enqueue(zb_backend *h, uint32_t kind, const uint32_t *lq, uint32_t k, const
         close_window(h);
         map();
     }
-    if (kind == dev::kWDense) {  // a relabelling may have put its qubits out of order
-        bool ascending = true;
-        for (uint32_t b = 1; b < k; ++b) ascending = ascending && qs[b] > qs[b - 1];
-        if (!ascending) {
-            flush_window(h);
-            return exact_unitary_T<T>(h, qs, k, src);
-        }
-    }
     if (h->win_ops.empty()) h->win_mask = low_mask(h);
     h->win_mask |= need;
     zb_backend::POp op{};
@@ -578,7 +571,9 @@ void enqueue(zb_backend *h, uint32_t kind, const uint32_t *lq, uint32_t k, const
 
 template <typename T>
 void flush_T(zb_backend *h) {
-    if (h->win_ops.size() == 1) {  // one op: the per-op kernel is the cheaper pass
+    // one op: the per-op kernel is the cheaper pass, except for a dense block
+    // on 3 or 4 qubits, whose per-op kernel is bound by its arithmetic
+    if (h->win_ops.size() == 1 && !(h->win_ops[0].kind == dev::kWDense && h->win_ops[0].k >= 3)) {
         h->wb_swaps.clear();
         const zb_backend::POp op = h->win_ops[0];
         std::vector<double> inter;
@@ -628,28 +623,67 @@ void flush_T(zb_backend *h) {
     };
     std::vector<dev::WinStage> stages;
     std::vector<uint32_t> stage_slots;  // bitmask of local bits per stage
-    uint32_t cur = 0;
+    std::vector<std::array<int, 64>> stage_index;  // local bit -> register bit, per stage
+    // A dense op's matrix bit b must sit on a lower register bit than its
+    // bit b + 1 (the register kernels walk a block's bits in that order), so
+    // a stage's slots are ordered: before[a] holds the local bits that must
+    // precede local bit a.  Windows with a 4-qubit block run 4 slots.
+    int ns = nslots(h);
+    for (const auto &p : h->win_ops)
+        if (p.kind == dev::kWDense) ns = std::max(ns, (int)p.k);
+    uint32_t cur = 0, before[dev::kMaxTileBits] = {};
     size_t begin = 0;
     bool dense = false;
-    const int ns = nslots(h);
+    // a topological order of the stage's slots (ties: lower local bit first)
+    // or an empty vector if the constraints have a cycle
+    auto order_of = [&](uint32_t sl, const uint32_t *bef) {
+        std::vector<int> ord;
+        uint32_t done = 0;
+        while (done != sl) {
+            int pick = -1;
+            for (uint32_t b = 0; b < m && pick < 0; ++b)
+                if (((sl & ~done) >> b) & 1 && (bef[b] & sl & ~done) == 0) pick = (int)b;
+            if (pick < 0) return std::vector<int>{};
+            ord.push_back(pick);
+            done |= 1u << pick;
+        }
+        return ord;
+    };
     auto close = [&](size_t end) {
         uint32_t sl = cur;
         for (uint32_t b = 0; b < m && __builtin_popcount(sl) < ns; ++b) sl |= 1u << b;
+        const std::vector<int> ord = order_of(sl, before);
         dev::WinStage st{};
-        int i = 0;
+        std::array<int, 64> index;
+        index.fill(-1);
+        for (int i = 0; i < (int)ord.size(); ++i) st.slot[i] = (uint32_t)ord[i], index[ord[i]] = i;
+        int j = 0;
         for (uint32_t b = 0; b < m; ++b)
-            if ((sl >> b) & 1) st.slot[i++] = b;
+            if ((sl >> b) & 1) st.ins[j++] = b;
         st.op_begin = (uint32_t)begin;
         st.op_end = (uint32_t)end;
         stages.push_back(st);
         stage_slots.push_back(sl);
+        stage_index.push_back(index);
     };
     for (size_t i = 0; i < h->win_ops.size(); ++i) {
-        const uint32_t nb = needs(h->win_ops[i]);
-        if (__builtin_popcount(cur | nb) > ns) {
+        const zb_backend::POp &p = h->win_ops[i];
+        const uint32_t nb = needs(p);
+        uint32_t bef[dev::kMaxTileBits];
+        std::copy(before, before + m, bef);
+        if (p.kind == dev::kWDense)
+            for (uint32_t b = 1; b < p.k; ++b)
+                for (uint32_t a = 0; a < b; ++a) bef[local[p.q[b]]] |= 1u << local[p.q[a]];
+        if (__builtin_popcount(cur | nb) > ns || order_of(cur | nb, bef).empty()) {
             close(i);
             begin = i;
             cur = 0;
+            std::fill(before, before + m, 0u);
+            if (p.kind == dev::kWDense)
+                for (uint32_t b = 1; b < p.k; ++b)
+                    for (uint32_t a = 0; a < b; ++a) before[local[p.q[b]]] |= 1u << local[p.q[a]];
+        } else {
+            std::copy(bef, bef + m, before);
         }
         cur |= nb;
     }
@@ -659,11 +693,7 @@ void flush_T(zb_backend *h) {
     std::vector<dev::WinOp> wops(h->win_ops.size());
     for (size_t si = 0; si < stages.size(); ++si) {
         const uint32_t sl = stage_slots[si];
-        auto slot_of = [&](int lb) {
-            int i = 0;
-            for (int b = 0; b < lb; ++b) i += (sl >> b) & 1;
-            return (uint32_t)i;
-        };
+        auto slot_of = [&](int lb) { return (uint32_t)stage_index[si][lb]; };
         for (uint32_t oi = stages[si].op_begin; oi < stages[si].op_end; ++oi) {
             const zb_backend::POp &p = h->win_ops[oi];
             dev::WinOp &w = wops[oi];
@@ -833,19 +863,16 @@ void discard_pending(zb_backend *h) {
 
 // The apply_* entry points take circuit qubits and hand state bits to the
 // kernels.  A dense op whose qubits land on state bits in a different order
-// keeps its matrix as written (the per-op kernel gathers in op order); only
-// windows need them ascending.
+// keeps its matrix as written: the per-op kernel gathers in op order, and a
+// window stage orders its register slots by the matrix's bits.  Dense ops on
+// up to 4 qubits join windows (a window holding a 4-qubit block runs with 4
+// slots); larger ones run the per-op kernels.
 template <typename T>
 void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
     uint32_t pq[dev::kMaxDiagK];
-    bool ascending = true;
     uint64_t qubits = 0;
-    for (uint32_t b = 0; b < k; ++b) {
-        pq[b] = h->phys[qs[b]];
-        ascending = ascending && (b == 0 || pq[b] > pq[b - 1]);
-        qubits |= 1ull << qs[b];
-    }
-    if (!tiled(h) || k > (uint32_t)nslots(h) || !ascending) {
+    for (uint32_t b = 0; b < k; ++b) qubits |= 1ull << qs[b];
+    if (!tiled(h) || k > (uint32_t)dev::kMaxSlots) {
         flush(h);  // held-back CX ops may close a relabelling window: map again
         for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
         return exact_unitary_T<T>(h, pq, k, mat);

@@ -466,8 +466,13 @@ struct WinOp {
 };
 
 struct WinStage {
-    uint32_t slot[kMaxSlots];  // ascending local bit positions (first NS used)
+    // register bit i of a lane's index <-> local bit slot[i] (first NS
+    // used, in any order: a dense op's matrix bits take increasing register
+    // bits); ins[] holds the same bits ascending, where the lane index
+    // takes zero bits
+    uint32_t slot[kMaxSlots];
     uint32_t op_begin, op_end;
+    uint32_t ins[kMaxSlots];
 };
 
 struct WinHeader {
@@ -839,7 +844,7 @@ __attribute__((amdgpu_waves_per_eu(NS == 3 && !kDense ? WinShape<T, TB, NS>::kWa
             for (int i = 0; i < NS; ++i) sl[i] = (int)st->slot[i];
             int lb = t;
 #pragma unroll
-            for (int i = 0; i < NS; ++i) lb += lb & -(1 << sl[i]);  // insert a zero bit at sl[i]
+            for (int i = 0; i < NS; ++i) lb += lb & -(1 << st->ins[i]);  // insert a zero bit at ins[i]
             const int plb = padded(lb);
             T xr[R], xi[R];
             // lb and the register bits are disjoint, so padded(lb | c) =
