@@ -866,13 +866,15 @@ void discard_pending(zb_backend *h) {
 // keeps its matrix as written: the per-op kernel gathers in op order, and a
 // window stage orders its register slots by the matrix's bits.  Dense ops on
 // up to 4 qubits join windows (a window holding a 4-qubit block runs with 4
-// slots); larger ones run the per-op kernels.
+// slots); larger ones, and 4-qubit ones under the knob "mfma", run the per-op
+// kernels.
 template <typename T>
 void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
     uint32_t pq[dev::kMaxDiagK];
     uint64_t qubits = 0;
     for (uint32_t b = 0; b < k; ++b) qubits |= 1ull << qs[b];
-    if (!tiled(h) || k > (uint32_t)dev::kMaxSlots) {
+    const bool on_mfma = h->mfma && k >= 4 && h->n >= (uint32_t)dev::kMfmaTileBits;  // knob "mfma"
+    if (!tiled(h) || k > (uint32_t)dev::kMaxSlots || on_mfma) {
         flush(h);  // held-back CX ops may close a relabelling window: map again
         for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
         return exact_unitary_T<T>(h, pq, k, mat);
