@@ -410,6 +410,24 @@ def test_20_qubit_counts_identical(orc):
         assert r.counts.total() == 1024 and r.backend == "hip-f64"
 
 
+def test_counts_with_scrambled_repeated_and_unmeasured_clbits(orc):
+    """Final measurements in any order into two registers: qubit q to clbit
+    (7q + 3) mod 18 of 20 clbits, qubit 5 measured a second time into clbit
+    0 (the later measure decides it), clbits 18 and 19 never written.  The
+    clbit words come from per-byte tables of the sample index; counts equal
+    the oracle's (exec.rs:442-447)."""
+    n = 17
+    rows = [orc.gen_random(n, 6, 808)]
+    rows += [orc.measure(q, (7 * q + 3) % 18) for q in range(n)] + [orc.measure(5, 0)]
+    prog = orc.program(rows)
+    cregs = [12, 8]
+    for shots, seed in ((4096, 3), (40000, 4)):
+        _say(f"scrambled clbits n={n} shots={shots}")
+        want, _ = orc.run(orc.Compiled(prog, n, cregs=cregs, fusion=1), shots, seed)
+        got = z.run_program(prog, n, cregs=cregs, opts=z.RunOptions(shots=shots, seed=seed, precision=64, fusion_max=1))
+        assert got.raw_counts == want
+
+
 @pytest.mark.parametrize("mode", MODES)
 def test_dynamic_and_noisy_runs_match_oracle(orc, mode):
     """Per-shot path: EXACT replays every shot through the zb_* calls,

@@ -3,6 +3,7 @@
 // src/noise.rs:91-192) driving the HIP backend only through the zb_* C ABI,
 // exactly as the reference's executor drives `dyn Backend`.
 #include <algorithm>
+#include <iterator>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -205,15 +206,25 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
                 // out strictly ascending (every qubit measured, clbits in
                 // qubit order) they are the result as they are, else they are
                 // sorted and equal words merged.
+                // The word is a bit gather (the last measure into a clbit
+                // decides it, clbits never measured stay 0), so it is the OR
+                // of one table entry per byte of the index.
+                int src[64];
+                std::fill(std::begin(src), std::end(src), -1);
+                for (const auto &fm : c.final_measures) src[fm.second] = (int)fm.first;
+                const uint32_t nbytes = (c.n_qubits + 7) / 8;
+                std::vector<uint64_t> lut((size_t)nbytes * 256, 0);
+                for (int cb = 0; cb < 64; ++cb)
+                    if (src[cb] >= 0)
+                        for (uint32_t v = 0; v < 256; ++v)
+                            if ((v >> (src[cb] & 7)) & 1) lut[(size_t)(src[cb] >> 3) * 256 + v] |= 1ull << cb;
                 bool ascending = true;
+                sampled.reserve(smp.size());
                 for (size_t i = 0; i < smp.size();) {
                     size_t j = i + 1;
                     while (j < smp.size() && smp[j] == smp[i]) ++j;
                     uint64_t cl = 0;
-                    for (const auto &fm : c.final_measures) {
-                        const uint64_t bit = (smp[i] >> fm.first) & 1;
-                        cl = (cl & ~(1ull << fm.second)) | (bit << fm.second);
-                    }
+                    for (uint32_t b = 0; b < nbytes; ++b) cl |= lut[(size_t)b * 256 + ((smp[i] >> (8 * b)) & 255)];
                     ascending = ascending && (sampled.empty() || sampled.back().first < cl);
                     sampled.emplace_back(cl, (uint64_t)(j - i));
                     i = j;
