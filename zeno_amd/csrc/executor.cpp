@@ -162,6 +162,17 @@ bool trivial(const zr_noise *m) {
                   m->readout_flip_1to0 == 0.0);
 }
 
+// The sampled path's host buffers (the draws' indices and the counts) are
+// kept per thread across runs: a fresh multi-megabyte vector costs a page
+// fault per 4 KiB on first touch, about 10 ms for 1M shots, more than the
+// counting itself.  A buffer past kKeepBytes is released after its run.
+constexpr size_t kKeepBytes = 64ull << 20;
+
+template <typename V>
+void trim(V &v) {
+    if (v.capacity() * sizeof(typename V::value_type) > kKeepBytes) V().swap(v);
+}
+
 bool valid(const zr_noise *m) {  // noise.rs:78-87
     if (!m) return true;
     for (double v : {m->depolarizing_1q, m->depolarizing_2q, m->bit_flip, m->phase_flip, m->amplitude_damping,
@@ -189,7 +200,9 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
     const int precision = zb_precision(hb);
     const zr_noise *nz = trivial(noise) ? nullptr : noise;
     std::unordered_map<uint64_t, uint64_t> raw;
-    std::vector<std::pair<uint64_t, uint64_t>> sampled;  // run_sampled's counts, ascending by word
+    // run_sampled's counts, ascending by word (a per-thread buffer: see kKeepBytes)
+    static thread_local std::vector<std::pair<uint64_t, uint64_t>> sampled;
+    sampled.clear();
     try {
         zb_backend *h = hb;
         ok(zb_set_mode(h, mode == 2 ? 1 : mode));  // 2: tiled, per-shot loop on the host
@@ -198,8 +211,10 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
             apply_ops_ahead(h, c.ops, 0, c.ops.size());
             ok(zb_finish(h));
             if (!c.final_measures.empty() && shots > 0) {
-                std::vector<uint64_t> smp(shots);
-                ok(zb_sample(h, shots, splitmix64(seed), smp.data()));
+                static thread_local std::vector<uint64_t> smp_buf;
+                if (smp_buf.size() < shots) smp_buf.resize(shots);
+                const uint64_t *smp = smp_buf.data();
+                ok(zb_sample(h, shots, splitmix64(seed), smp_buf.data()));
                 // Counts per basis index (exec.rs:437-441): the samples come
                 // back ascending, so equal indices are runs.  Then each unique
                 // index's clbit word (exec.rs:442-447); when the words come
@@ -219,10 +234,10 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
                         for (uint32_t v = 0; v < 256; ++v)
                             if ((v >> (src[cb] & 7)) & 1) lut[(size_t)(src[cb] >> 3) * 256 + v] |= 1ull << cb;
                 bool ascending = true;
-                sampled.reserve(smp.size());
-                for (size_t i = 0; i < smp.size();) {
+                sampled.reserve(shots);
+                for (size_t i = 0; i < shots;) {
                     size_t j = i + 1;
-                    while (j < smp.size() && smp[j] == smp[i]) ++j;
+                    while (j < shots && smp[j] == smp[i]) ++j;
                     uint64_t cl = 0;
                     for (uint32_t b = 0; b < nbytes; ++b) cl |= lut[(size_t)b * 256 + ((smp[i] >> (8 * b)) & 255)];
                     ascending = ascending && (sampled.empty() || sampled.back().first < cl);
@@ -238,6 +253,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
                     }
                     sampled.resize(m);
                 }
+                trim(smp_buf);
             }
         } else {  // run_per_shot (exec.rs:459-549)
             bool batched = false;
@@ -281,6 +297,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
     }
     if (!nz && !c.dynamic) {
         out.swap(sampled);
+        trim(sampled);
         return ZB_OK;
     }
     out.assign(raw.begin(), raw.end());
@@ -452,7 +469,8 @@ int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t se
         set_error("invalid noise model: every probability must be in [0, 1]");
         return ZB_ERR_ARG;
     }
-    std::vector<std::pair<uint64_t, uint64_t>> v;
+    static thread_local std::vector<std::pair<uint64_t, uint64_t>> v;  // see kKeepBytes
+    v.clear();
     const auto t0 = std::chrono::steady_clock::now();
     zb_backend *hp = nullptr;
     if (const int rc = zb_create(cc->c.n_qubits, precision, &hp); rc < 0) return rc;
@@ -464,7 +482,9 @@ int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t se
         return ZB_ERR_ARG;
     }
     for (size_t i = 0; i < v.size(); ++i) keys[i] = v[i].first, counts[i] = v[i].second;
-    return (int64_t)v.size();
+    const int64_t m = (int64_t)v.size();
+    trim(v);
+    return m;
 }
 
 }  // extern "C"

@@ -1246,6 +1246,14 @@ __global__ __launch_bounds__(256) void k_chunk_sums_multi(const T *__restrict__ 
 // chunk from cum_before[c], emitting the sorted draws us[s0..s1) that fall
 // below the running sum; leftovers land on the chunk's last index with
 // nonzero probability.
+//
+// Per slab, lane 0 forms the running sums in index order (the reference's
+// serial chain, so the same bits) and writes them over the slab's values in
+// LDS.  The reference emits draw u at the first index whose running sum
+// exceeds u; the running sums never decrease (each step adds r*r + i*i >= 0
+// under monotonic rounding), so the slab takes exactly the draws below its
+// last running sum, and every lane places its share of them by binary
+// search for that first index.
 template <typename T, typename S>
 __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, Layout L,
                                                    uint64_t len,
@@ -1255,6 +1263,8 @@ __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, 
     __shared__ double tv[S::kSpan];
     __shared__ uint64_t sptr;
     __shared__ int sdone;
+    __shared__ double stop;
+    __shared__ int srun;
     __shared__ uint64_t slast[S::kBlock];
     // touched lists the chunks with draws; without it block c is chunk c,
     // and a chunk without draws has nothing to do
@@ -1281,44 +1291,55 @@ __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, 
         __syncthreads();
         if (S::kPrefetch && slab + S::kSpan < hi) w.load(re, im, L, slab + S::kSpan, count(slab + S::kSpan));
         if (threadIdx.x == 0) {
-            // The next four draws' thresholds stay in registers (+inf past
-            // the chunk's last draw): an emission shifts them and loads the
-            // one four ahead, so a run of emissions does not wait on memory.
-            // acc never decreases (it adds r*r + i*i >= 0 and rounding is
-            // monotonic), so a group whose last running sum does not pass
-            // the next threshold emits nothing and needs no per-element test.
-            uint64_t ptr = sptr;
-            auto draw = [&](uint64_t i) { return i < s1 ? us[i] : __builtin_inf(); };
-            double nu = draw(ptr), nu1 = draw(ptr + 1), nu2 = draw(ptr + 2), nu3 = draw(ptr + 3);
-            auto emit = [&](int j, double x) {
-                while (nu < x) {
-                    out[ptr++] = slab + (uint64_t)j;
-                    nu = nu1;
-                    nu1 = nu2;
-                    nu2 = nu3;
-                    nu3 = draw(ptr + 3);
-                }
-            };
+            // a group is read before its running sums overwrite it; the walk
+            // stops once the running sum passes the chunk's last draw
+            const double ulast = us[s1 - 1];
+            int run = cnt;
             serial_walk<S::kGroup>(
                 tv, cnt,
                 [&](int g, const double *v) {
-                    double run[S::kGroup], x = acc;
+                    double x = acc;
 #pragma unroll
-                    for (int u = 0; u < S::kGroup; ++u) run[u] = x = x + v[u];
+                    for (int u = 0; u < S::kGroup; ++u) tv[g + u] = x = x + v[u];
                     acc = x;
-                    if (nu < x) {
-#pragma unroll
-                        for (int u = 0; u < S::kGroup; ++u) emit(g + u, run[u]);
-                    }
-                    return ptr < s1;
+                    if (x <= ulast) return true;
+                    run = g + S::kGroup;
+                    return false;
                 },
                 [&](int j, double v) {
-                    acc += v;
-                    emit(j, acc);
-                    return ptr < s1;
+                    tv[j] = acc = acc + v;
+                    if (acc <= ulast) return true;
+                    run = j + 1;
+                    return false;
                 });
-            sptr = ptr;
-            sdone = ptr == s1;
+            stop = acc;
+            srun = run;
+        }
+        __syncthreads();
+        const double top = stop;
+        const int run = srun;
+        uint64_t first_out = s1;  // the lane's first draw at or above top
+        for (uint64_t i = sptr + threadIdx.x; i < s1; i += S::kBlock) {
+            const double u = us[i];
+            if (!(u < top)) {
+                first_out = i;
+                break;
+            }
+            int a = 0, b = run - 1;  // tv[run - 1] = top > u
+            while (a < b) {
+                const int mid = (a + b) >> 1;
+                if (tv[mid] > u) b = mid;
+                else a = mid + 1;
+            }
+            out[i] = slab + (uint64_t)a;
+        }
+        slast[threadIdx.x] = first_out;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint64_t m = s1;
+            for (int t = 0; t < S::kBlock; ++t) m = slast[t] < m ? slast[t] : m;
+            sptr = m;
+            sdone = m == s1;
         }
         __syncthreads();
         if (sdone) break;
