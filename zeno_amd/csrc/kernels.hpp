@@ -50,6 +50,10 @@ __device__ __forceinline__ uint64_t phys_index(const Layout &L, uint64_t l) {
     return L.lut[l & 255] | L.lut[256 + ((l >> 8) & 255)] | L.lut[512 + ((l >> 16) & 255)] |
            L.lut[768 + ((l >> 24) & 255)] | L.lut[1024 + ((l >> 32) & 255)];
 }
+// The layout permutes index bits, so it distributes over disjoint bit sets:
+// phys_index(a | b) = phys_index(a) | phys_index(b).  Walks over logical
+// indices split each index into a lane part (looked up once per lane) and a
+// part shared by the workgroup (the same entries for every lane).
 
 __device__ __forceinline__ uint64_t gtid() { return (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; }
 __device__ __forceinline__ uint64_t gstride() { return (uint64_t)gridDim.x * blockDim.x; }
@@ -404,13 +408,21 @@ __global__ __launch_bounds__(kBlock) void k_reduce_state(const T *__restrict__ r
                                                          double *__restrict__ out) {
     __shared__ double s[kBlock];
     const uint64_t base = (uint64_t)blockIdx.x * kReduceSpan;
-    const uint64_t qs = 1ull << q;
+    // logical index insert_zero(e, q) | 2^q (mode 1) or e, where e = base +
+    // j*kBlock + lane: insert_zero maps every bit on its own, so the lane's
+    // bits, the block's, row j's (bits 8..11 of e, at most bit 12 after the
+    // insertion: one byte-1 entry of the layout) and the q bit are disjoint
+    static_assert(kBlock == 256 && kReduceSpan == 16 * kBlock, "row j holds bits 8..11 of e");
+    const uint64_t t = threadIdx.x;
+    const uint64_t lane = phys_index(L, mode ? insert_zero(t, q) : t) | (mode ? phys_index(L, 1ull << q) : 0);
+    const uint64_t blk = phys_index(L, mode ? insert_zero(base, q) : base);
     double a = 0.0;
 #pragma unroll
     for (int j = 0; j < kReduceSpan / kBlock; ++j) {
-        const uint64_t e = base + (uint64_t)j * kBlock + threadIdx.x;
-        if (e < n_elems) {
-            const uint64_t idx = phys_index(L, mode ? insert_zero(e, q) + qs : e);
+        const uint64_t jr = (uint64_t)j * kBlock;
+        if (base + jr + t < n_elems) {
+            const uint64_t jl = mode ? insert_zero(jr, q) : jr;
+            const uint64_t idx = lane | blk | (L.lut ? L.lut[256 + (jl >> 8)] : jl);
             const double r = (double)re[idx], m = (double)im[idx];
             a = a + (r * r + m * m);
         }
@@ -1101,13 +1113,16 @@ template <typename T, typename S>
 struct SampSlab {
     static constexpr int kPerLane = S::kSpan / S::kBlock;
     T r[kPerLane], m[kPerLane];
+    // Slabs start at multiples of S::kSpan (or 0 for a state shorter than
+    // one), so slab, the lane and u * S::kBlock are disjoint index parts.
     __device__ __forceinline__ void load(const T *__restrict__ re, const T *__restrict__ im, const Layout &L,
                                          uint64_t slab, int cnt) {
+        const uint64_t lane = phys_index(L, threadIdx.x) | phys_index(L, slab);
 #pragma unroll
         for (int u = 0; u < kPerLane; ++u) {
             const int j = threadIdx.x + u * S::kBlock;
             if (j < cnt) {
-                const uint64_t p = phys_index(L, slab + j);
+                const uint64_t p = lane | phys_index(L, (uint64_t)u * S::kBlock);
                 r[u] = re[p];
                 m[u] = im[p];
             }
@@ -1116,8 +1131,9 @@ struct SampSlab {
     // Without prefetch: load, square and store one value at a time.
     __device__ __forceinline__ static void fill(const T *__restrict__ re, const T *__restrict__ im, const Layout &L,
                                                 uint64_t slab, int cnt, double *tv) {
+        const uint64_t lane = phys_index(L, threadIdx.x) | phys_index(L, slab);
         for (int j = threadIdx.x; j < cnt; j += S::kBlock) {
-            const uint64_t p = phys_index(L, slab + j);
+            const uint64_t p = lane | phys_index(L, (uint64_t)(j - (int)threadIdx.x));
             const double a = (double)re[p], b = (double)im[p];
             tv[j] = a * a + b * b;
         }
