@@ -26,6 +26,9 @@
 namespace zeno {
 // sort.hip: the sampler's draws sorted and routed to chunks on the device
 size_t draw_sort_temp_bytes(uint64_t n);
+size_t draw_jump_bytes(uint64_t shots);
+hipError_t draws_on_device(const uint64_t *state, uint64_t shots, double total, void *jump_dev,
+                           std::vector<uint64_t> &host, double *us, hipStream_t stream);
 hipError_t draw_sort_and_route(void *temp, size_t temp_bytes, const double *us, double *sorted, uint64_t n,
                                const double *cum, uint64_t m, uint64_t *starts, hipStream_t stream);
 }  // namespace zeno
@@ -1020,7 +1023,8 @@ size_t sample_scratch_bytes(uint64_t len, uint64_t shots) {
     if (shots >= kDeviceSortShots && shots <= kDeviceSortMaxShots) {
         const size_t tb = (draw_sort_temp_bytes(shots) + 255) & ~size_t(255);
         const size_t nb = ((nch + 1) * 8 + 255) & ~size_t(255), sb = (shots * 8 + 255) & ~size_t(255);
-        need = std::max(need, tb + 2 * nb + 3 * sb + 256);
+        const size_t jb = (draw_jump_bytes(shots) + 255) & ~size_t(255);
+        need = std::max(need, tb + 2 * nb + 3 * sb + jb + 256);
     } else {
         need = std::max<size_t>(need, std::min(nch, shots) * 4 + (nch + 1) * 8 * 2 + shots * 8 * 2 + 64);
     }
@@ -1047,13 +1051,12 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out, char 
     for (uint64_t c = 0; c < nch; ++c) cum[c + 1] = cum[c] + sums[c];
     const double total = cum[nch];
     Rng rng(seed);
-    std::vector<double> us(shots);
-    for (uint64_t i = 0; i < shots; ++i) us[i] = rng.f64() * total;
     if (shots >= kDeviceSortShots && shots <= kDeviceSortMaxShots) {
-        // sort, route and rescan on the device; every chunk gets a block
+        // draw, sort, route and rescan on the device; every chunk gets a block
         const size_t tb = (draw_sort_temp_bytes(shots) + 255) & ~size_t(255);
         const size_t nb = ((nch + 1) * 8 + 255) & ~size_t(255), sb = (shots * 8 + 255) & ~size_t(255);
-        char *p = get_scratch(tb + 2 * nb + 3 * sb + 256);
+        const size_t jb = (draw_jump_bytes(shots) + 255) & ~size_t(255);
+        char *p = get_scratch(tb + 2 * nb + 3 * sb + jb + 256);
         p = (char *)(((uintptr_t)p + 255) & ~(uintptr_t)255);
         void *temp = p;
         double *dcum = (double *)(p + tb);
@@ -1061,14 +1064,18 @@ void sample_T(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out, char 
         double *dus = (double *)(p + tb + 2 * nb);
         double *dsorted = (double *)(p + tb + 2 * nb + sb);
         uint64_t *dout = (uint64_t *)(p + tb + 2 * nb + 2 * sb);
+        void *djump = p + tb + 2 * nb + 3 * sb;
+        std::vector<uint64_t> staging;  // read by the copies below until the final sync
         check(hipMemcpyAsync(dcum, cum.data(), (nch + 1) * 8, hipMemcpyHostToDevice, h->stream), "cum");
-        check(hipMemcpyAsync(dus, us.data(), shots * 8, hipMemcpyHostToDevice, h->stream), "us");
+        check(draws_on_device(rng.state(), shots, total, djump, staging, dus, h->stream), "k_draws");
         check(draw_sort_and_route(temp, tb, dus, dsorted, shots, dcum, nch + 1, dstarts, h->stream), "draw sort");
         rescan_T<T>(h, nch, chunk, nullptr, dcum, dstarts, dsorted, dout);
         check(hipMemcpyAsync(out, dout, shots * 8, hipMemcpyDeviceToHost, h->stream), "out copy");
         h->sync();
         return;
     }
+    std::vector<double> us(shots);
+    for (uint64_t i = 0; i < shots; ++i) us[i] = rng.f64() * total;
     std::sort(us.begin(), us.end());
     std::vector<uint64_t> starts(nch + 1);
     for (uint64_t c = 0; c <= nch; ++c)

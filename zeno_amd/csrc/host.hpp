@@ -33,6 +33,7 @@ class Rng {
     uint64_t next();
     double f64();
     uint64_t range(uint64_t lo, uint64_t hi);
+    const uint64_t *state() const { return s_; }  // the next output's state (s0..s3)
 
   private:
     uint64_t s_[4];
