@@ -1264,23 +1264,27 @@ __global__ __launch_bounds__(256) void k_chunk_sums_multi(const T *__restrict__ 
 // nonzero probability.
 //
 // Per slab, lane 0 forms the running sums in index order (the reference's
-// serial chain, so the same bits) and writes them over the slab's values in
-// LDS.  The reference emits draw u at the first index whose running sum
-// exceeds u; the running sums never decrease (each step adds r*r + i*i >= 0
-// under monotonic rounding), so the slab takes exactly the draws below its
-// last running sum, and every lane places its share of them by binary
-// search for that first index.
+// serial chain, so the same bits) and keeps the one at the end of each
+// group of S::kGroup values in LDS.  The reference emits draw u at the first
+// index whose running sum exceeds u; the running sums never decrease (each
+// step adds r*r + i*i >= 0 under monotonic rounding), so the slab takes
+// exactly the draws below its last running sum, and every lane places its
+// share of them: a binary search over the group ends finds the group, and
+// the group's sums, recomputed from the previous group's end with the same
+// adds in the same order, give the index.
 template <typename T, typename S>
 __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, const T *__restrict__ im, Layout L,
                                                    uint64_t len,
                                                    uint64_t chunk, const uint32_t *__restrict__ touched,
                                                    const double *__restrict__ cum, const uint64_t *__restrict__ starts,
                                                    const double *__restrict__ us, uint64_t *__restrict__ out) {
+    constexpr int kGroups = S::kSpan / S::kGroup;
     __shared__ double tv[S::kSpan];
+    __shared__ double gend[kGroups];  // running sum at the end of each group
     __shared__ uint64_t sptr;
     __shared__ int sdone;
-    __shared__ double stop;
-    __shared__ int srun;
+    __shared__ double sstart;         // running sum before the slab
+    __shared__ int srun;              // values walked in the slab
     __shared__ uint64_t slast[S::kBlock];
     // touched lists the chunks with draws; without it block c is chunk c,
     // and a chunk without draws has nothing to do
@@ -1307,47 +1311,55 @@ __global__ __launch_bounds__(S::kBlock) void k_rescan(const T *__restrict__ re, 
         __syncthreads();
         if (S::kPrefetch && slab + S::kSpan < hi) w.load(re, im, L, slab + S::kSpan, count(slab + S::kSpan));
         if (threadIdx.x == 0) {
-            // a group is read before its running sums overwrite it; the walk
-            // stops once the running sum passes the chunk's last draw
+            // the walk stops once the running sum passes the chunk's last draw
             const double ulast = us[s1 - 1];
             int run = cnt;
+            sstart = acc;
             serial_walk<S::kGroup>(
                 tv, cnt,
                 [&](int g, const double *v) {
                     double x = acc;
 #pragma unroll
-                    for (int u = 0; u < S::kGroup; ++u) tv[g + u] = x = x + v[u];
+                    for (int u = 0; u < S::kGroup; ++u) x = x + v[u];
                     acc = x;
+                    gend[g / S::kGroup] = x;
                     if (x <= ulast) return true;
                     run = g + S::kGroup;
                     return false;
                 },
                 [&](int j, double v) {
-                    tv[j] = acc = acc + v;
+                    acc = acc + v;
+                    gend[j / S::kGroup] = acc;  // the partial last group ends at its last value
                     if (acc <= ulast) return true;
                     run = j + 1;
                     return false;
                 });
-            stop = acc;
             srun = run;
         }
         __syncthreads();
-        const double top = stop;
-        const int run = srun;
-        uint64_t first_out = s1;  // the lane's first draw at or above top
+        const int run = srun, ngroups = (run + S::kGroup - 1) / S::kGroup;
+        const double top = gend[ngroups - 1];  // the running sum after the last value walked
+        uint64_t first_out = s1;               // the lane's first draw at or above top
         for (uint64_t i = sptr + threadIdx.x; i < s1; i += S::kBlock) {
             const double u = us[i];
             if (!(u < top)) {
                 first_out = i;
                 break;
             }
-            int a = 0, b = run - 1;  // tv[run - 1] = top > u
+            int a = 0, b = ngroups - 1;  // gend[ngroups - 1] = top > u
             while (a < b) {
                 const int mid = (a + b) >> 1;
-                if (tv[mid] > u) b = mid;
+                if (gend[mid] > u) b = mid;
                 else a = mid + 1;
             }
-            out[i] = slab + (uint64_t)a;
+            double x = a > 0 ? gend[a - 1] : sstart;
+            int j = a * S::kGroup;
+            const int jend = j + S::kGroup < run ? j + S::kGroup : run;
+            for (; j < jend - 1; ++j) {
+                x = x + tv[j];
+                if (x > u) break;
+            }
+            out[i] = slab + (uint64_t)j;
         }
         slast[threadIdx.x] = first_out;
         __syncthreads();
