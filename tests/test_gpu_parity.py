@@ -363,6 +363,25 @@ def test_sampler_many_chunks_bit_exact(orc):
         orc.set_threads(1)
 
 
+def test_sampler_device_draws_at_block_and_jump_edges(orc):
+    """From 32768 shots the draws come from the device: threads of 256
+    draws each, started by jump maps over the bits of the thread index.  At
+    the first device shot count, one past it, a count that ends inside a
+    block, one whose thread count needs another jump level, and 2^20 + 3,
+    the indices equal the oracle's sequential stream (host path below)."""
+    n = 14
+    prog = random_mixed(orc, n, 3 * n, 1414)
+    for prec in (64, 32):
+        o = orc.State(n, prec)
+        o.apply_compiled(orc.Compiled(prog, n, fusion=2))
+        with z.HipBackend(n, prec) as g:
+            g.apply_compiled(z.compile(prog, n, fusion=2))
+            for shots, seed in ((32767, 1), (32768, 2), (32769, 3), (65536 + 100, 4), (256 * 513, 5),
+                                ((1 << 20) + 3, 6)):
+                _say(f"device draws prec={prec} shots={shots}")
+                assert np.array_equal(g.sample(shots, seed), o.sample(shots, seed)), (prec, shots)
+
+
 def test_sampler_concentrated_distributions_bit_exact(orc):
     """Mass on a few indices: GHZ (index 0 and the last index, every chunk
     between empty) and H on qubits 0..2 only (8 indices inside the first
