@@ -202,6 +202,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
     std::unordered_map<uint64_t, uint64_t> raw;
     // run_sampled's counts, ascending by word (a per-thread buffer: see kKeepBytes)
     static thread_local std::vector<std::pair<uint64_t, uint64_t>> sampled;
+    trim(sampled);  // an earlier run that failed may have left a large buffer
     sampled.clear();
     try {
         zb_backend *h = hb;
@@ -212,6 +213,7 @@ int run_compiled_on(zb_backend *hb, const zr_compiled *cc, uint64_t shots, uint6
             ok(zb_finish(h));
             if (!c.final_measures.empty() && shots > 0) {
                 static thread_local std::vector<uint64_t> smp_buf;
+                trim(smp_buf);
                 if (smp_buf.size() < shots) smp_buf.resize(shots);
                 const uint64_t *smp = smp_buf.data();
                 ok(zb_sample(h, shots, splitmix64(seed), smp_buf.data()));
@@ -470,6 +472,7 @@ int64_t zr_run(const zr_compiled *cc, int precision, uint64_t shots, uint64_t se
         return ZB_ERR_ARG;
     }
     static thread_local std::vector<std::pair<uint64_t, uint64_t>> v;  // see kKeepBytes
+    trim(v);
     v.clear();
     const auto t0 = std::chrono::steady_clock::now();
     zb_backend *hp = nullptr;
