@@ -106,9 +106,10 @@ int zb_reset_all(zb_backend *h);
 
 /* Backend::sample (src/exec.rs:154; src/sample.rs:29-111) with
  * rng = Xoshiro256PlusPlus::seed_from_u64(seed).  Writes `shots` basis
- * indices in ascending order to `out` (host memory).  The handle's device
- * scratch grows to hold the draws (about 24 bytes per shot from 32768
- * shots, when they are sorted on the device). */
+ * indices in ascending order to `out` (host memory).  From 32768 shots the
+ * draws (the same xoshiro256++ stream) are generated, sorted and routed on
+ * the device, and the handle's device scratch grows to hold them (about 24
+ * bytes per shot, plus up to 160 KiB of the stream's jump maps). */
 int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out);
 
 /* zb_sample without any device allocation.  No counterpart in the
