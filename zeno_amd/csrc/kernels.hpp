@@ -106,14 +106,13 @@ __global__ __launch_bounds__(kBlock) void k_1q(T *__restrict__ re, T *__restrict
 // kern_kq_scalar (src/state.rs:355-392): gather 2^K amplitudes of a group,
 // row-major matvec with ar = (ar + wr*xr) - wi*xi, ai = (ai + wr*xi) + wi*xr
 // over ascending columns, scatter.  The matrix ([re | im], 2*4^K of T) is
-// staged in LDS; every lane reads the same element (broadcast).
+// the same for every lane: its elements are read with scalar loads into
+// scalar registers and enter the arithmetic from there.
 template <typename T, int K, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict__ im, const T *__restrict__ mat,
                                                Qubits qs, uint64_t groups) {
     constexpr int D = 1 << K;
-    __shared__ T sm[2 * D * D];
-    for (int i = threadIdx.x; i < 2 * D * D; i += blockDim.x) sm[i] = mat[i];
-    __syncthreads();
+    const T *__restrict__ sm = mat;
     uint64_t bit[K];
 #pragma unroll
     for (int b = 0; b < K; ++b) bit[b] = 1ull << qs.q[b];
