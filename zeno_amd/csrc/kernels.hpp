@@ -130,7 +130,7 @@ __global__ __launch_bounds__(kBlock) void k_kq(T *__restrict__ re, T *__restrict
             xr[j] = re[o];
             xi[j] = im[o];
         }
-        constexpr int kRowUnroll = K <= 4 ? D : 1;
+        constexpr int kRowUnroll = K <= 3 ? D : 1;
 #pragma unroll kRowUnroll
         for (int r = 0; r < D; ++r) {
             T ar = T(0), ai = T(0);
