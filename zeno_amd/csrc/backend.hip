@@ -434,12 +434,13 @@ void exact_diag_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *d
     std::copy(di.begin(), di.end(), packed.begin() + dim);
     const T *tab = static_cast<const T *>(h->upload(packed.data(), packed.size() * sizeof(T)));
     const dev::Qubits q = make_qubits(qs, k);
+    const size_t lds = 2 * (size_t)dim * sizeof(T);
     h->timed(ZB_K_DIAGK, h->sweep_bytes(), [&] {
         if (h->fma)
-            hipLaunchKernelGGL((dev::k_diagk<T, true>), dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream,
+            hipLaunchKernelGGL((dev::k_diagk<T, true>), dim3(grid_for(h->len)), dim3(dev::kBlock), lds, h->stream,
                                h->R<T>(), h->I<T>(), tab, q, h->len);
         else
-            hipLaunchKernelGGL((dev::k_diagk<T, false>), dim3(grid_for(h->len)), dim3(dev::kBlock), 0, h->stream,
+            hipLaunchKernelGGL((dev::k_diagk<T, false>), dim3(grid_for(h->len)), dim3(dev::kBlock), lds, h->stream,
                                h->R<T>(), h->I<T>(), tab, q, h->len);
     });
     check(hipGetLastError(), "k_diagk");

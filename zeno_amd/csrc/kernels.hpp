@@ -349,17 +349,29 @@ __global__ __launch_bounds__(kBlock) void k_diag1(T *__restrict__ re, T *__restr
 template <typename T, bool F = false>
 __global__ __launch_bounds__(kBlock) void k_diagk(T *__restrict__ re, T *__restrict__ im, const T *__restrict__ tab,
                                                   Qubits qs, uint64_t len) {
-    __shared__ T st[2 << kMaxDiagK];
+    // the table ([re | im], 2 * 2^k values) in dynamic LDS of its own size,
+    // so small tables leave room for more workgroups per CU
+    extern __shared__ __align__(16) unsigned char diag_lds[];
+    T *const st = reinterpret_cast<T *>(diag_lds);
     const int dim = 1 << qs.k;
-    for (int i = threadIdx.x; i < dim; i += blockDim.x) {
-        st[i] = tab[i];
-        st[(1 << kMaxDiagK) + i] = tab[dim + i];
-    }
+    for (int i = threadIdx.x; i < 2 * dim; i += blockDim.x) st[i] = tab[i];
     __syncthreads();
-    for (uint64_t i = gtid(); i < len; i += gstride()) {
-        uint32_t j = 0;
-        for (int b = 0; b < qs.k; ++b) j |= (uint32_t)((i >> qs.q[b]) & 1) << b;
-        const T wr = st[j], wi = st[(1 << kMaxDiagK) + j];
+    // Element i = row + t: a workgroup's rows are multiples of kBlock, so the
+    // table bits on qubits below log2(kBlock) come from the lane (once) and
+    // the others from the row (uniform per iteration).
+    static_assert(kBlock == 256, "lane bits are index bits 0..7");
+    const uint32_t t = threadIdx.x;
+    uint32_t jl = 0;
+    for (int b = 0; b < qs.k; ++b)
+        if (qs.q[b] < 8) jl |= ((t >> qs.q[b]) & 1) << b;
+    for (uint64_t row = (uint64_t)blockIdx.x * kBlock; row < len; row += gstride()) {
+        uint32_t jr = 0;
+        for (int b = 0; b < qs.k; ++b)
+            if (qs.q[b] >= 8) jr |= (uint32_t)((row >> qs.q[b]) & 1) << b;
+        const uint64_t i = row + t;
+        if (i >= len) break;
+        const uint32_t j = jl | jr;
+        const T wr = st[j], wi = st[dim + j];
         const T ar = re[i], ai = im[i];
         re[i] = msc<F>(wr * ar, wi, ai);
         im[i] = mac<F>(wr * ai, wi, ar);
