@@ -109,7 +109,7 @@ int zb_reset_all(zb_backend *h);
  * indices in ascending order to `out` (host memory).  From 32768 shots the
  * draws (the same xoshiro256++ stream) are generated, sorted and routed on
  * the device, and the handle's device scratch grows to hold them (about 24
- * bytes per shot, plus up to 160 KiB of the stream's jump maps). */
+ * bytes per shot, plus up to 176 KiB of the stream's jump maps). */
 int zb_sample(zb_backend *h, uint64_t shots, uint64_t seed, uint64_t *out);
 
 /* zb_sample without any device allocation.  No counterpart in the
