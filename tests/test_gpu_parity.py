@@ -89,6 +89,28 @@ def test_single_ops_bit_exact(orc, prec):
     g.close()
 
 
+@pytest.mark.parametrize("prec", [64, 32])
+def test_exact_mode_diagonals_bit_exact(orc, prec):
+    """The per-op diagonal kernels (exact mode): tables of 1 to 12 bits on
+    qubits below and above index bit 8 (the table index splits into a lane
+    part and a row part there), and a state shorter than one workgroup."""
+    rng = np.random.default_rng(17)
+    for n, ks in ((5, (1, 2, 3, 5)), (14, (1, 2, 4, 7, 9, 12))):
+        g = z.HipBackend(n, prec, z.HipBackend.EXACT)
+        o = orc.State(n, prec)
+        prog = orc.gen_random(n, 2, 50 + n)
+        g.apply_compiled(z.compile(prog, n, fusion=1))
+        o.apply_compiled(orc.Compiled(prog, n, fusion=1))
+        for k in ks:
+            for qs in (sorted(rng.choice(n, k, replace=False).tolist()), list(range(n - k, n))):
+                d = np.exp(1j * rng.uniform(0, 2 * np.pi, 1 << k)) * rng.uniform(0.5, 1.5, 1 << k)
+                _say(f"exact diag prec={prec} n={n} k={k} qs={qs}")
+                g.apply_diagonal(qs, d)
+                o.apply_diagonal(qs, d)
+        _assert_same(g, o, f"exact diagonals n={n}")
+        g.close()
+
+
 MODES = [z.HipBackend.EXACT, z.HipBackend.TILED]
 
 
