@@ -259,6 +259,58 @@ MFMA_TOL = {64: 1e-12, 32: 1e-5}
 
 
 @pytest.mark.parametrize("prec", [64, 32])
+def test_dense_ops_on_lds_tiles_bit_exact(orc, prec):
+    """Dense 4..6-qubit ops through LDS tiles (knob kq_tiled=2; by default
+    from 12 qubits where the per-lane gather is slow) and gathered per lane
+    (kq_tiled=0) both give the oracle's kern_kq bits: op qubits low, high,
+    low and high mixed, spread, under a shuffled qubit layout (the op's
+    state bits out of order), and at 17 qubits where each workgroup walks
+    several tiles.  Under the knob fma=1 the two data paths still agree bit
+    for bit with each other."""
+    rng = np.random.default_rng(8)
+    for n, layout in ((12, 0), (14, 0), (17, 0), (14, 5)):
+        re, im = _rand_state(n, prec, 60 + n)
+        o = orc.State(n, prec)
+        o.set_raw(re, im)
+        gs = {}
+        for tiled in (2, 0):
+            g = z.HipBackend(n, prec, z.HipBackend.EXACT)
+            g.tune("layout", layout)
+            g.tune("kq_tiled", tiled)
+            g.set_raw(re, im)
+            gs[tiled] = g
+        for k in (4, 5, 6):
+            for qs in (list(range(k)), list(range(n - k, n)), [0, 1, 2, n - 3, n - 2, n - 1][:k],
+                       sorted(rng.choice(n, k, replace=False).tolist())):
+                m = _unitary(rng, k)
+                _say(f"kq tiles prec={prec} n={n} layout={layout} k={k} qs={qs}")
+                o.apply_unitary(qs, m)
+                for tiled, g in gs.items():
+                    g.apply_unitary(qs, m)
+                    _assert_same(g, o, f"n={n} layout={layout} k={k} qs={qs} kq_tiled={tiled}")
+        for g in gs.values():
+            g.close()
+    n = 14
+    re, im = _rand_state(n, prec, 99)
+    gs = {}
+    for tiled in (2, 0):
+        g = z.HipBackend(n, prec, z.HipBackend.EXACT)
+        g.tune("fma", 1)
+        g.tune("kq_tiled", tiled)
+        g.set_raw(re, im)
+        gs[tiled] = g
+    for k, qs in ((4, [0, 5, 9, 13]), (5, [0, 1, 2, 3, 4]), (6, [0, 2, 7, 8, 11, 13])):
+        m = _unitary(rng, k)
+        _say(f"kq tiles fma prec={prec} k={k} qs={qs}")
+        for g in gs.values():
+            g.apply_unitary(qs, m)
+    (ar, ai), (br, bi) = gs[2].raw(), gs[0].raw()
+    assert np.array_equal(_bits(ar), _bits(br)) and np.array_equal(_bits(ai), _bits(bi))
+    for g in gs.values():
+        g.close()
+
+
+@pytest.mark.parametrize("prec", [64, 32])
 def test_mfma_dense_ops_within_tolerance(orc, prec):
     """Dense 4..6-qubit ops on the matrix cores (knob mfma=1) against the
     oracle's scalar kern_kq on the same state: op qubits on the low run

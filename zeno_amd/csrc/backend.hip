@@ -131,6 +131,9 @@ struct zb_backend {
     std::vector<std::pair<uint32_t, uint32_t>> wb_swaps;  // state-bit swaps of the window being flushed
     bool lut_dirty = false;
     bool mfma = false;  // dense 4..6-qubit ops on the matrix cores (within tolerance)
+    // dense 4..6-qubit ops through LDS tiles (k_kq's results): 0 never,
+    // 1 where the per-lane gather is slow, 2 always
+    int kq_tiled = 1;
     bool fma = false;   // fused multiply-adds in the sweep arithmetic (within tolerance)
     std::vector<unsigned char> win_data;    // 1q / dense matrices (copied into LDS)
     std::vector<unsigned char> win_tables;  // diagonal tables (read from global)
@@ -370,6 +373,66 @@ void launch_kq_mfma(zb_backend *h, const std::vector<T> &mr, const std::vector<T
     check(hipGetLastError(), "k_kq_mfma");
 }
 
+// Column swizzle of the exact tiled kernel: the op bits on a wave's lane
+// bits (tile bits 0..5) take, in order, the column bits 0..5 that no group
+// bit there takes, so the 64 lanes of a transposing LDS access see 64
+// distinct columns mod 64.
+void kq_swizzle(dev::MfmaTile &t) {
+    uint32_t used = 0;
+    for (int i = 0; i < 6; ++i)
+        if (!(t.role[i] & dev::kRoleOp)) used |= 1u << t.role[i];
+    int next = 0;
+    for (int i = 0; i < 6; ++i) {
+        if (!(t.role[i] & dev::kRoleOp)) continue;
+        while ((used >> next) & 1) ++next;
+        t.swz[t.role[i] & 7] = 1u << next;
+        used |= 1u << next;
+    }
+}
+
+// The exact dense kernel on LDS tiles: the matrix packed per wave and
+// column, [wave][column][row of the wave's 2^K/8][re, im].
+template <typename T, int K>
+void launch_kq_tiled(zb_backend *h, const std::vector<T> &mr, const std::vector<T> &mi, const uint32_t *pq) {
+    constexpr int D = 1 << K, RW = D / 8;
+    std::vector<T> pk((size_t)2 * D * D);
+    for (int w = 0; w < 8; ++w)
+        for (int c = 0; c < D; ++c)
+            for (int rr = 0; rr < RW; ++rr) {
+                const size_t o = (((size_t)w * D + c) * RW + rr) * 2;
+                pk[o] = mr[(size_t)(RW * w + rr) * D + c];
+                pk[o + 1] = mi[(size_t)(RW * w + rr) * D + c];
+            }
+    const T *dp = static_cast<const T *>(h->upload(pk.data(), pk.size() * sizeof(T)));
+    dev::MfmaTile t = mfma_tile(h, pq, K);
+    kq_swizzle(t);
+    const uint64_t tiles = h->len >> dev::kMfmaTileBits;
+    const unsigned grid = (unsigned)std::min<uint64_t>(tiles, 1024);
+    h->timed(ZB_K_KQ + K, h->sweep_bytes(), [&] {
+        if (h->fma)
+            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, true>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream,
+                               h->R<T>(), h->I<T>(), dp, t, tiles);
+        else
+            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, false>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream,
+                               h->R<T>(), h->I<T>(), dp, t, tiles);
+    });
+    check(hipGetLastError(), "k_kq_tiled");
+}
+
+// Whether a dense op on state bits pq runs on LDS tiles.  The per-lane
+// gather of k_kq is up to 10% faster while at most two op bits lie inside
+// a 128-byte run (its loads stay close to coalesced); with three it is
+// 1.4x slower and with five 3.6x.  In f64 a 6-qubit group (256 registers
+// of amplitudes) does not fit one lane well, and tiles are 3.6x faster.
+bool use_kq_tiled(const zb_backend *h, const uint32_t *pq, uint32_t k) {
+    if (h->kq_tiled == 0 || k < 4 || h->n < (uint32_t)dev::kMfmaTileBits) return false;
+    if (h->kq_tiled == 2) return true;
+    const uint32_t run = h->prec == 64 ? 4 : 5;
+    uint32_t low = 0;
+    for (uint32_t b = 0; b < k; ++b) low += pq[b] < run;
+    return low >= 3 || (h->prec == 64 && k == 6);
+}
+
 template <typename T>
 void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double *mat) {
     const size_t dim = (size_t)1 << k;
@@ -380,6 +443,13 @@ void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
             case 4: return launch_kq_mfma<T, 4>(h, mr, mi, qs);
             case 5: return launch_kq_mfma<T, 5>(h, mr, mi, qs);
             case 6: return launch_kq_mfma<T, 6>(h, mr, mi, qs);
+        }
+    }
+    if (use_kq_tiled(h, qs, k)) {
+        switch (k) {
+            case 4: return launch_kq_tiled<T, 4>(h, mr, mi, qs);
+            case 5: return launch_kq_tiled<T, 5>(h, mr, mi, qs);
+            case 6: return launch_kq_tiled<T, 6>(h, mr, mi, qs);
         }
     }
     if (k == 1) {
@@ -1277,7 +1347,9 @@ void reset_knobs(zb_backend *h) {
     h->defer_cx = d.defer_cx;
     h->relabel = d.relabel;
     h->mfma = d.mfma;
+    h->kq_tiled = d.kq_tiled;
     h->fma = d.fma;
+    h->chunk_multi = d.chunk_multi;
 }
 
 void set_shape(zb_backend *h, uint32_t n_qubits, int precision) {
@@ -1344,6 +1416,7 @@ int apply_knob(zb_backend *h, const std::string &k, int64_t value) {
     else if (k == "tile_bits" && (value == 0 || value == 12 || value == 13)) h->tile_bits = (uint32_t)value;
     else if (k == "defer_cx") h->defer_cx = value != 0;
     else if (k == "mfma") h->mfma = value != 0;
+    else if (k == "kq_tiled" && value >= 0 && value <= 2) h->kq_tiled = (int)value;
     else if (k == "fma") h->fma = value != 0;
     else if (k == "relabel") h->relabel = value != 0;
     else if (k == "chunk_multi" && value >= 0) h->chunk_multi = (uint64_t)value;
@@ -1748,7 +1821,7 @@ int zb_tune(zb_backend *h, const char *key, int64_t value) {
 
 int zb_tune_default(uint32_t n_qubits, int precision, const char *key, int64_t value) {
     static const char *keys[] = {"prefetch", "win_grid", "slots", "page_bits", "tile_bits",
-                                 "defer_cx", "layout",   "mfma",  "fma",       "relabel", "chunk_multi"};
+                                 "defer_cx", "layout",   "mfma",  "fma",       "relabel", "chunk_multi", "kq_tiled"};
     const std::string k(key ? key : "");
     if (std::find(std::begin(keys), std::end(keys), k) == std::end(keys) || (precision != 32 && precision != 64)) {
         set_error("zb_tune_default: unknown key or precision: " + k);
