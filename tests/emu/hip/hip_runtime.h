@@ -1,7 +1,8 @@
 // Test infrastructure: host stand-ins for the HIP built-ins that the batched
-// trajectory kernel (zeno_amd/csrc/trajectories.hpp) uses, so that the
-// kernel's own source runs on the CPU under sanitizers.  A workgroup's 256
-// lanes are cooperative contexts on one thread (tests/emu/lanes.hpp):
+// trajectory kernel (zeno_amd/csrc/trajectories.hpp) and the exact tiled
+// dense kernel (zeno_amd/csrc/kq_tiles.hpp) use, so that the kernels' own
+// source runs on the CPU under sanitizers.  A workgroup's lanes are
+// cooperative contexts on one thread (tests/emu/lanes.hpp):
 // __syncthreads hands control back to the scheduler, which resumes every
 // lane to its next barrier, in an order of the test's choosing, before any
 // passes it.  __shared__ arrays are function statics (one workgroup runs at
@@ -38,6 +39,9 @@ void barrier();  // tests/emu/lanes.hpp
 #endif
 
 inline void __syncthreads() { emu::barrier(); }
+
+// A value the kernel already holds uniform across its wave: every lane's own.
+#define __builtin_amdgcn_readfirstlane(x) (x)
 
 inline uint64_t __umul64hi(uint64_t a, uint64_t b) {
     return (uint64_t)(((unsigned __int128)a * (unsigned __int128)b) >> 64);

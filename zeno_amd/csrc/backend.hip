@@ -327,29 +327,6 @@ void launch_kq(zb_backend *h, const T *dmat, const dev::Qubits &qs) {
     check(hipGetLastError(), "k_kq");
 }
 
-// Tile of the matrix-core kernel for an op on state bits pq (op order):
-// the op's bits, the low bits of a 128-byte run, then the lowest free bits,
-// ascending; X row bit b <-> op bit b, the other tile bits index groups.
-dev::MfmaTile mfma_tile(const zb_backend *h, const uint32_t *pq, uint32_t k) {
-    uint64_t m = 0;
-    for (uint32_t b = 0; b < k; ++b) m |= 1ull << pq[b];
-    const uint32_t run = h->prec == 64 ? 4 : 5;
-    m |= (1ull << run) - 1;
-    for (uint32_t b = 0; __builtin_popcountll(m) < dev::kMfmaTileBits; ++b) m |= 1ull << b;
-    dev::MfmaTile t{};
-    uint32_t i = 0, g = 0;
-    for (uint32_t b = 0; b < 64; ++b) {
-        if (!((m >> b) & 1)) continue;
-        t.tq[i] = b;
-        t.role[i] = g;
-        for (uint32_t j = 0; j < k; ++j)
-            if (pq[j] == b) t.role[i] = dev::kRoleOp | j;
-        if (!(t.role[i] & dev::kRoleOp)) ++g;
-        ++i;
-    }
-    return t;
-}
-
 template <typename T, int K>
 void launch_kq_mfma(zb_backend *h, const std::vector<T> &mr, const std::vector<T> &mi, const uint32_t *pq) {
     constexpr int D = 1 << K, D2 = 2 * D;
@@ -363,57 +340,32 @@ void launch_kq_mfma(zb_backend *h, const std::vector<T> &mr, const std::vector<T
             w[(D + r) * D2 + D + c] = a;
         }
     const T *dw = static_cast<const T *>(h->upload(w.data(), w.size() * sizeof(T)));
-    const dev::MfmaTile t = mfma_tile(h, pq, K);
-    const uint64_t tiles = h->len >> dev::kMfmaTileBits;
+    const dev::KqTile t = kq_tile(h->prec == 64 ? 4 : 5, pq, K);
+    const uint64_t tiles = h->len >> dev::kKqTileBits;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, 1024);
     h->timed(ZB_K_KQ_MFMA + K, h->sweep_bytes(), [&] {
-        hipLaunchKernelGGL((dev::k_kq_mfma<T, K>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream, h->R<T>(),
+        hipLaunchKernelGGL((dev::k_kq_mfma<T, K>), dim3(grid), dim3(dev::kKqThreads), 0, h->stream, h->R<T>(),
                            h->I<T>(), dw, t, tiles);
     });
     check(hipGetLastError(), "k_kq_mfma");
-}
-
-// Column swizzle of the exact tiled kernel: the op bits on a wave's lane
-// bits (tile bits 0..5) take, in order, the column bits 0..5 that no group
-// bit there takes, so the 64 lanes of a transposing LDS access see 64
-// distinct columns mod 64.
-void kq_swizzle(dev::MfmaTile &t) {
-    uint32_t used = 0;
-    for (int i = 0; i < 6; ++i)
-        if (!(t.role[i] & dev::kRoleOp)) used |= 1u << t.role[i];
-    int next = 0;
-    for (int i = 0; i < 6; ++i) {
-        if (!(t.role[i] & dev::kRoleOp)) continue;
-        while ((used >> next) & 1) ++next;
-        t.swz[t.role[i] & 7] = 1u << next;
-        used |= 1u << next;
-    }
 }
 
 // The exact dense kernel on LDS tiles: the matrix packed per wave and
 // column, [wave][column][row of the wave's 2^K/8][re, im].
 template <typename T, int K>
 void launch_kq_tiled(zb_backend *h, const std::vector<T> &mr, const std::vector<T> &mi, const uint32_t *pq) {
-    constexpr int D = 1 << K, RW = D / 8;
-    std::vector<T> pk((size_t)2 * D * D);
-    for (int w = 0; w < 8; ++w)
-        for (int c = 0; c < D; ++c)
-            for (int rr = 0; rr < RW; ++rr) {
-                const size_t o = (((size_t)w * D + c) * RW + rr) * 2;
-                pk[o] = mr[(size_t)(RW * w + rr) * D + c];
-                pk[o + 1] = mi[(size_t)(RW * w + rr) * D + c];
-            }
+    const std::vector<T> pk = kq_tiled_pack<T>(K, mr.data(), mi.data());
     const T *dp = static_cast<const T *>(h->upload(pk.data(), pk.size() * sizeof(T)));
-    dev::MfmaTile t = mfma_tile(h, pq, K);
+    dev::KqTile t = kq_tile(h->prec == 64 ? 4 : 5, pq, K);
     kq_swizzle(t);
-    const uint64_t tiles = h->len >> dev::kMfmaTileBits;
+    const uint64_t tiles = h->len >> dev::kKqTileBits;
     const unsigned grid = (unsigned)std::min<uint64_t>(tiles, 1024);
     h->timed(ZB_K_KQ + K, h->sweep_bytes(), [&] {
         if (h->fma)
-            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, true>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream,
+            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, true>), dim3(grid), dim3(dev::kKqThreads), 0, h->stream,
                                h->R<T>(), h->I<T>(), dp, t, tiles);
         else
-            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, false>), dim3(grid), dim3(dev::kMfmaThreads), 0, h->stream,
+            hipLaunchKernelGGL((dev::k_kq_tiled<T, K, false>), dim3(grid), dim3(dev::kKqThreads), 0, h->stream,
                                h->R<T>(), h->I<T>(), dp, t, tiles);
     });
     check(hipGetLastError(), "k_kq_tiled");
@@ -425,7 +377,7 @@ void launch_kq_tiled(zb_backend *h, const std::vector<T> &mr, const std::vector<
 // 1.4x slower and with five 3.6x.  In f64 a 6-qubit group (256 registers
 // of amplitudes) does not fit one lane well, and tiles are 3.6x faster.
 bool use_kq_tiled(const zb_backend *h, const uint32_t *pq, uint32_t k) {
-    if (h->kq_tiled == 0 || k < 4 || h->n < (uint32_t)dev::kMfmaTileBits) return false;
+    if (h->kq_tiled == 0 || k < 4 || h->n < (uint32_t)dev::kKqTileBits) return false;
     if (h->kq_tiled == 2) return true;
     const uint32_t run = h->prec == 64 ? 4 : 5;
     uint32_t low = 0;
@@ -438,7 +390,7 @@ void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     const size_t dim = (size_t)1 << k;
     std::vector<T> mr, mi;
     to_T<T>(mat, dim * dim, mr, mi);
-    if (h->mfma && k >= 4 && h->n >= (uint32_t)dev::kMfmaTileBits) {
+    if (h->mfma && k >= 4 && h->n >= (uint32_t)dev::kKqTileBits) {
         switch (k) {
             case 4: return launch_kq_mfma<T, 4>(h, mr, mi, qs);
             case 5: return launch_kq_mfma<T, 5>(h, mr, mi, qs);
@@ -947,7 +899,7 @@ void apply_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     uint32_t pq[dev::kMaxDiagK];
     uint64_t qubits = 0;
     for (uint32_t b = 0; b < k; ++b) qubits |= 1ull << qs[b];
-    const bool on_mfma = h->mfma && k >= 4 && h->n >= (uint32_t)dev::kMfmaTileBits;  // knob "mfma"
+    const bool on_mfma = h->mfma && k >= 4 && h->n >= (uint32_t)dev::kKqTileBits;  // knob "mfma"
     if (!tiled(h) || k > (uint32_t)dev::kMaxSlots || on_mfma) {
         flush(h);  // held-back CX ops may close a relabelling window: map again
         for (uint32_t b = 0; b < k; ++b) pq[b] = h->phys[qs[b]];
