@@ -212,9 +212,10 @@ int zb_layout(const zb_backend *h, uint32_t *phys);
  * from this many 2^16-value chunks the sampler's pass 1 walks 8 chunks per
  * workgroup instead of one; 0 = never; the sums are the same serial
  * chains either way) and "kq_tiled" (0/1/2, default 1: per-op dense ops on
- * 4 to 6 qubits, n >= 12, stage tiles of 2^12 amplitudes through LDS
- * instead of gathering each group into one lane: 0 never, 1 when three
- * or more op bits lie inside a 128-byte run or for 6 qubits in f64, 2
+ * 2 to 6 qubits, n >= 12, stage tiles of 2^12 amplitudes through LDS
+ * instead of gathering each group into one lane: 0 never, 1 when
+ * min(k, 3) or more op bits lie inside a 128-byte run or for 6 qubits in
+ * f64, 2
  * always; the same products and sums in the same order either way).  None
  * of them changes results, except
  * "mfma" (0/1, default 0): dense ops on 4 to 6 qubits run as real GEMMs on

@@ -55,6 +55,8 @@ int run(uint32_t n, int k, const std::vector<uint32_t> &qs, unsigned grid, int o
     const uint64_t tiles = len >> dev::kKqTileBits;
     T *re = st.data(), *im = st.data() + len;
     switch (k) {
+        case 2: launch<T, 2>(re, im, pk.data(), t, tiles, grid, order); break;
+        case 3: launch<T, 3>(re, im, pk.data(), t, tiles, grid, order); break;
         case 4: launch<T, 4>(re, im, pk.data(), t, tiles, grid, order); break;
         case 5: launch<T, 5>(re, im, pk.data(), t, tiles, grid, order); break;
         case 6: launch<T, 6>(re, im, pk.data(), t, tiles, grid, order); break;

@@ -260,7 +260,7 @@ MFMA_TOL = {64: 1e-12, 32: 1e-5}
 
 @pytest.mark.parametrize("prec", [64, 32])
 def test_dense_ops_on_lds_tiles_bit_exact(orc, prec):
-    """Dense 4..6-qubit ops through LDS tiles (knob kq_tiled=2; by default
+    """Dense 2..6-qubit ops through LDS tiles (knob kq_tiled=2; by default
     from 12 qubits where the per-lane gather is slow) and gathered per lane
     (kq_tiled=0) both give the oracle's kern_kq bits: op qubits low, high,
     low and high mixed, spread, under a shuffled qubit layout (the op's
@@ -279,7 +279,7 @@ def test_dense_ops_on_lds_tiles_bit_exact(orc, prec):
             g.tune("kq_tiled", tiled)
             g.set_raw(re, im)
             gs[tiled] = g
-        for k in (4, 5, 6):
+        for k in (2, 3, 4, 5, 6):
             for qs in (list(range(k)), list(range(n - k, n)), [0, 1, 2, n - 3, n - 2, n - 1][:k],
                        sorted(rng.choice(n, k, replace=False).tolist())):
                 m = _unitary(rng, k)
@@ -299,7 +299,7 @@ def test_dense_ops_on_lds_tiles_bit_exact(orc, prec):
         g.tune("kq_tiled", tiled)
         g.set_raw(re, im)
         gs[tiled] = g
-    for k, qs in ((4, [0, 5, 9, 13]), (5, [0, 1, 2, 3, 4]), (6, [0, 2, 7, 8, 11, 13])):
+    for k, qs in ((2, [0, 13]), (3, [0, 1, 2]), (4, [0, 5, 9, 13]), (5, [0, 1, 2, 3, 4]), (6, [0, 2, 7, 8, 11, 13])):
         m = _unitary(rng, k)
         _say(f"kq tiles fma prec={prec} k={k} qs={qs}")
         for g in gs.values():

@@ -53,7 +53,7 @@ def _cases():
     rng = np.random.default_rng(21)
     out = []
     for prec in (64, 32):
-        for k in (4, 5, 6):
+        for k in (2, 3, 4, 5, 6):
             out += [(prec, 12, list(range(k)), None, 1, 0),
                     (prec, 13, list(range(13 - k, 13)), None, 2, 1),
                     (prec, 14, [0, 1, 2, 11, 12, 13][:k], None, 3, 2),

@@ -131,7 +131,7 @@ struct zb_backend {
     std::vector<std::pair<uint32_t, uint32_t>> wb_swaps;  // state-bit swaps of the window being flushed
     bool lut_dirty = false;
     bool mfma = false;  // dense 4..6-qubit ops on the matrix cores (within tolerance)
-    // dense 4..6-qubit ops through LDS tiles (k_kq's results): 0 never,
+    // dense 2..6-qubit ops through LDS tiles (k_kq's results): 0 never,
     // 1 where the per-lane gather is slow, 2 always
     int kq_tiled = 1;
     bool fma = false;   // fused multiply-adds in the sweep arithmetic (within tolerance)
@@ -374,15 +374,17 @@ void launch_kq_tiled(zb_backend *h, const std::vector<T> &mr, const std::vector<
 // Whether a dense op on state bits pq runs on LDS tiles.  The per-lane
 // gather of k_kq is up to 10% faster while at most two op bits lie inside
 // a 128-byte run (its loads stay close to coalesced); with three it is
-// 1.4x slower and with five 3.6x.  In f64 a 6-qubit group (256 registers
-// of amplitudes) does not fit one lane well, and tiles are 3.6x faster.
+// 1.4x (5 qubits) to 2.6x (3 qubits) slower, and with five 3.6x; a
+// 2-qubit op is tiled when both its bits lie in the run (1.7x).  In f64 a
+// 6-qubit group (256 registers of amplitudes) does not fit one lane well,
+// and tiles are 3.6x faster.
 bool use_kq_tiled(const zb_backend *h, const uint32_t *pq, uint32_t k) {
-    if (h->kq_tiled == 0 || k < 4 || h->n < (uint32_t)dev::kKqTileBits) return false;
+    if (h->kq_tiled == 0 || k < 2 || h->n < (uint32_t)dev::kKqTileBits) return false;
     if (h->kq_tiled == 2) return true;
     const uint32_t run = h->prec == 64 ? 4 : 5;
     uint32_t low = 0;
     for (uint32_t b = 0; b < k; ++b) low += pq[b] < run;
-    return low >= 3 || (h->prec == 64 && k == 6);
+    return low >= std::min(k, 3u) || (h->prec == 64 && k == 6);
 }
 
 template <typename T>
@@ -399,6 +401,8 @@ void exact_unitary_T(zb_backend *h, const uint32_t *qs, uint32_t k, const double
     }
     if (use_kq_tiled(h, qs, k)) {
         switch (k) {
+            case 2: return launch_kq_tiled<T, 2>(h, mr, mi, qs);
+            case 3: return launch_kq_tiled<T, 3>(h, mr, mi, qs);
             case 4: return launch_kq_tiled<T, 4>(h, mr, mi, qs);
             case 5: return launch_kq_tiled<T, 5>(h, mr, mi, qs);
             case 6: return launch_kq_tiled<T, 6>(h, mr, mi, qs);

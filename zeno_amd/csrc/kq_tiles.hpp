@@ -1,4 +1,4 @@
-// kq_tiles.hpp — dense ops on 4 to 6 qubits over LDS tiles: a workgroup of
+// kq_tiles.hpp — dense ops on 2 to 6 qubits over LDS tiles: a workgroup of
 // 512 lanes takes 2^12 amplitudes whose 12 state bits are the op's K bits,
 // the low bits of a 128-byte run and the lowest free bits, i.e. G =
 // 2^(12-K) whole groups of the op.  Holds the tile walk that the exact VALU
@@ -135,30 +135,34 @@ __device__ __forceinline__ void kq_tile_loop(T *__restrict__ re, T *__restrict__
 
 // ------------------------------------------------------ kq through LDS
 // kern_kq_scalar (src/state.rs:355-392) on the matrix-core kernel's tiles,
-// for K = 4..6: the same products and sums as k_kq, in the same order (per
+// for K = 2..6: the same products and sums as k_kq, in the same order (per
 // output row, ascending columns, from 0), so the result is bit for bit
 // k_kq's; what changes is the data path.  k_kq gathers a group's 2^K
 // amplitudes into one lane's registers, which leaves loads uncoalesced when
 // the op's bits are low and needs 4*2^K registers of amplitudes.  Here the
-// tile is read and written in whole 128-byte runs and transposed in LDS;
-// wave w computes output rows [RW*w, RW*w + RW) (RW = D/8) of the G groups
-// (G/64 per lane, lane l holds groups l + 64i), reading X column by column.
+// tile is read and written in whole 128-byte runs and transposed in LDS.
+// Wave w computes the RW = max(D/8, 1) output rows of row band w / WS over
+// a slice of G/WS groups (WS = max(8/D, 1) waves share a band when D < 8),
+// 8 outputs per lane (lane l holds the slice's groups l + 64i), reading X
+// column by column.
 // X rows are not padded: the op bits that lie on a wave's lane bits XOR
 // column bits that no group bit there uses (swz), so the transposing LDS
 // accesses of a wave fall on distinct columns mod 64 whichever bits the op
-// has.  The matrix is packed per wave and column ([w][c][row][re, im]), so
-// the 2*RW weights of one column are one uniform run read by scalar loads.
+// has.  The matrix is packed per row band and column ([band][c][row][re,
+// im]), so the 2*RW weights of one column are one uniform run read by
+// scalar loads.
 template <typename T, int K, bool F>
 __global__ __launch_bounds__(kKqThreads) void k_kq_tiled(T *__restrict__ re, T *__restrict__ im,
                                                            const T *__restrict__ pk, KqTile mt, uint64_t tiles) {
     constexpr int TB = kKqTileBits;
     constexpr int D = 1 << K, G = 1 << (TB - K), SP = G;
-    constexpr int RW = D / 8, GL = G / 64;
-    static_assert(RW * GL == 8 && RW >= 1 && GL >= 1, "tile shape");
+    constexpr int RW = D >= 8 ? D / 8 : 1, WS = D >= 8 ? 1 : 8 / D, GL = G / 64 / WS;
+    static_assert(RW * GL == 8 && GL >= 1, "tile shape");
     __shared__ T sx[2 * D * SP];
     const int ln = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const T *__restrict__ wp = pk + (size_t)wv * D * 2 * RW;
+    const int band = wv / WS, g0 = (wv % WS) * (G / WS);  // row band, first group of the slice
+    const T *__restrict__ wp = pk + (size_t)band * D * 2 * RW;
     auto fsw = [&](int j) {  // column XOR of X row j (< 64, uniform)
         int f = 0;
 #pragma unroll
@@ -178,8 +182,8 @@ __global__ __launch_bounds__(kKqThreads) void k_kq_tiled(T *__restrict__ re, T *
             T xr[GL], xi[GL];
 #pragma unroll
             for (int gi = 0; gi < GL; ++gi) {
-                xr[gi] = sx[c * SP + col + 64 * gi];
-                xi[gi] = sx[(D + c) * SP + col + 64 * gi];
+                xr[gi] = sx[c * SP + g0 + col + 64 * gi];
+                xi[gi] = sx[(D + c) * SP + g0 + col + 64 * gi];
             }
             const T *__restrict__ wc = wp + c * 2 * RW;
 #pragma unroll
@@ -195,11 +199,11 @@ __global__ __launch_bounds__(kKqThreads) void k_kq_tiled(T *__restrict__ re, T *
         __syncthreads();  // every wave has read X
 #pragma unroll
         for (int rr = 0; rr < RW; ++rr) {
-            const int row = RW * wv + rr, col = ln ^ fsw(row);
+            const int row = RW * band + rr, col = ln ^ fsw(row);
 #pragma unroll
             for (int gi = 0; gi < GL; ++gi) {
-                sx[row * SP + col + 64 * gi] = ar[rr][gi];
-                sx[(D + row) * SP + col + 64 * gi] = ai[rr][gi];
+                sx[row * SP + g0 + col + 64 * gi] = ar[rr][gi];
+                sx[(D + row) * SP + g0 + col + 64 * gi] = ai[rr][gi];
             }
         }
     });
@@ -248,13 +252,14 @@ inline void kq_swizzle(dev::KqTile &t) {
     }
 }
 
-// The exact tiled kernel's matrix, packed per wave and column:
-// [wave][column][row of the wave's 2^K/8][re, im] from row-major mr, mi.
+// The exact tiled kernel's matrix, packed per row band and column:
+// [band][column][row of the band's max(2^K/8, 1)][re, im] from row-major
+// mr, mi.
 template <typename T>
 std::vector<T> kq_tiled_pack(int K, const T *mr, const T *mi) {
-    const int D = 1 << K, RW = D / 8;
+    const int D = 1 << K, RW = D >= 8 ? D / 8 : 1;
     std::vector<T> pk((size_t)2 * D * D);
-    for (int w = 0; w < 8; ++w)
+    for (int w = 0; w < D / RW; ++w)
         for (int c = 0; c < D; ++c)
             for (int rr = 0; rr < RW; ++rr) {
                 const size_t o = (((size_t)w * D + c) * RW + rr) * 2;
