@@ -29,9 +29,10 @@ struct KqTile {
 };
 
 // Streams every tile of 2^12 amplitudes through LDS as X[2D][SP] and back:
-// op() reads X and, after a barrier of its own, writes Y over it.  Element
-// (row j, group g) sits at j*SP + (g ^ f(j)), f(j) the XOR of swz[b] over
-// the op bits b set in j (f = 0 keeps the plain layout).  The next tile's
+// op() reads X and, after a barrier of its own, writes Y over it.  The re
+// of element j of group g sits at j*SP + (g ^ f(j)) and its im at (D + j)*SP
+// + (g ^ f(j)), f(j) the XOR of swz[b] over the op bits b set in j (f = 0
+// keeps the plain layout).  The next tile's
 // loads are issued before op() and the previous tile's stores after this
 // tile's loads landed, so a wave keeps one tile in flight each way while it
 // computes.
